@@ -1,0 +1,146 @@
+/*
+ * pd_oracle.h -- CPU oracle for the bond-associated NOSB peridynamics step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or the timed CPU baseline.  The product (lib2dpdgpu, include/pd_hip.h) never
+ * links, loads or calls it.
+ *
+ * Two restatements of the reference algorithm live here:
+ *
+ *  ref_cpu_*   A literal restatement of src/main_seq.cpp (EricYJA/2D_PD_GPU):
+ *              the same O(N*M^3) loop nest of computeVelocity
+ *              (src/main_seq.cpp:396-470) -> computeForceDensityStates
+ *              (:271-380) -> computeShapeTensors (:71-156) ->
+ *              computeStressTensor (:204-268), in x87 long double with the
+ *              reference's double/long-double promotions, and the matrix
+ *              arithmetic of src/matrix.cpp:19-129 (unguarded inverse2D).
+ *              Single particle range per call so it can be timed on a bounded
+ *              sample.  Parity: pinned against output/velocity_step_0_after_
+ *              seq_ref.txt (Hassa mesh, 6 printed digits) and against
+ *              or_forces_fp64 on small meshes.
+ *
+ *  or_*        The restructured O(N*M^2) FP64 oracle in displacement form
+ *              (u = y - X, bond term du = u_l - u_i).  K_ik and N_ik do not
+ *              depend on j (src/main_seq.cpp:325), the weight is symmetric, so
+ *              each family is evaluated once.  Adds the project's bond-breaking
+ *              rule (critical stretch, both directions, FP64) and the
+ *              scale-relative singular-K rule, which the reference lacks.
+ *              Parity: pinned against ref_cpu_* (tiny meshes, all digits the
+ *              formulations share), against the Hassa fixtures, and by the
+ *              closed forms in tests/test_oracle.py (affine patch test, rigid
+ *              motion, momentum balance, weight special cases, Verlet closed
+ *              form).
+ */
+#ifndef PD_ORACLE_H
+#define PD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Material / discretisation parameters shared by every or_* routine. */
+typedef struct {
+    double n1, n2;     /* weight exponents (src/main_seq.cpp:609) */
+    double horizon;    /* delta in units of dx                     */
+    double dx;         /* grid spacing                              */
+    double rho;        /* mass density                              */
+    double dt;         /* step size                                 */
+    double E;          /* Young's modulus (used when E_arr == NULL) */
+    double nu;         /* Poisson ratio                             */
+    double s0;         /* critical stretch; <= 0 disables breaking  */
+    double kappa_tol;  /* singular K: det(K) <= kappa_tol*tr(K)^2   */
+} or_params;
+
+/* Weight omega_i(j,k) of src/main_seq.cpp:103-118 for bonds xj, xk (initial
+ * bond vectors from the family centre). */
+double or_weight(const or_params* p, double xj0, double xj1, double xk0, double xk1);
+
+/* Reverse-bond index: rev[b] is the bond (j->i) for b = (i->j).  Returns 0 on
+ * success, -1 when some bond has no reverse (asymmetric families). */
+int or_build_rev(int n, const int* row, const int* col, int* rev);
+
+/* Force density f_i = sum_j (T_ij - T_ji) V_j for displacement u (2n).
+ * broken: per-bond flag (may be NULL).  E_arr: per-particle E (may be NULL).
+ * T_out (2*nb, may be NULL) receives T_ij per bond.  Kinv_out (3*nb, may be
+ * NULL) receives the symmetric K_ik^-1 (xx, xy, yy).
+ * bugcompat != 0 reproduces the reference GPU kernel's current-bond vector
+ * y_l - X_i (include/mechanics.cuh:165): du_l := u_l instead of u_l - u_i.
+ * Returns the number of bonds whose K was treated as singular. */
+long or_forces_fp64(const or_params* p, int n, const double* X, const double* V,
+                    const double* E_arr, const int* row, const int* col,
+                    const int* rev, const unsigned char* broken, const double* u,
+                    int bugcompat, double* f_out, double* T_out, double* Kinv_out,
+                    int nthreads);
+
+/* Stretch of bond (i->j): (|xi + du| - |xi|) / |xi|, evaluated in FP64 with
+ * no fused operations, identical for both directions of a bond. */
+double or_bond_stretch(double xi0, double xi1, double du0, double du1);
+
+/* Mark bonds whose stretch exceeds s0 as broken (irreversible).  Returns the
+ * number of newly broken directed bonds. */
+long or_break_bonds(const or_params* p, int n, const double* X, const int* row,
+                    const int* col, const double* u, unsigned char* broken);
+
+/* Full explicit steps (project step order, reference semantics for the BC
+ * handling of src/main_seq.cpp:680-699):
+ *   velocity BC -> a = f/rho; u += v dt + 0.5 a dt^2 -> fixed BC ->
+ *   bond breaking -> forces -> v += 0.5 (a + f/rho) dt.
+ * bc[i]: -1 free, -2 fixed (u reset to 0), g >= 0 velocity group with
+ * prescribed velocity v0[2g..2g+1]. */
+long or_step_fp64(const or_params* p, int n, const double* X, const double* V,
+                  const double* E_arr, const int* row, const int* col, const int* rev,
+                  const int* bc, const double* v0, double* u, double* v, double* a,
+                  double* f, unsigned char* broken, int nsteps, int nthreads);
+
+/* ---------------- literal long-double restatement of main_seq -------------- */
+
+/* Parameters exactly as the reference passes them: n1, n2, horizon, dx and
+ * E, nu are double at the call sites (src/main_seq.cpp:396, :180); massDensity
+ * and stepSize are long double. */
+typedef struct {
+    double n1, n2, horizon, dx;
+    double E, nu;
+    long double rho, dt;
+} ref_params;
+
+/* computeVelocity (src/main_seq.cpp:396-470) for particles [i0, i1):
+ * X, y: 2n initial / current positions; V: n volumes; families as CSR in the
+ * reference's neighbour order.  Updates v[2i..] and f[2i..] (netF) for
+ * i in [i0, i1); reads a.  nthreads > 1 splits the range (results identical,
+ * each particle is independent). */
+void ref_cpu_compute_velocity(const ref_params* p, int n, const long double* X,
+                              const long double* y, const long double* V,
+                              const int* row, const int* col, long double* v,
+                              const long double* a, long double* f, int i0, int i1,
+                              int nthreads);
+
+/* updatePositions (src/main_seq.cpp:386-393). */
+void ref_cpu_update_positions(const ref_params* p, int n, long double* y,
+                              const long double* v, long double* a,
+                              const long double* f);
+
+/* applyVelocityBC (src/main_seq.cpp:28-36). */
+void ref_cpu_apply_velocity_bc(int cnt, const int* ids, const long double* v0,
+                               long double* v, long double* a);
+
+/* applyFixedBC (src/main_seq.cpp:38-46). */
+void ref_cpu_apply_fixed_bc(int cnt, const int* ids, const long double* X,
+                            long double* y);
+
+/* findNeighbor (src/SetupParticleSystem.cpp:100-136), all-pairs in long
+ * double with the cutoff |xi| < 1.01*dx*horizon.  Writes CSR (row: n+1,
+ * col: capacity cap) in ascending neighbour order.  Returns nb, or -1 when
+ * cap is too small. */
+long ref_find_neighbor(int n, const long double* X, long double dx,
+                       long double horizon, int* row, int* col, long cap);
+
+/* 2x2 long-double matrix helpers of src/matrix.cpp, exposed so that they can
+ * be checked bit-for-bit against the reference's own matrix.cpp. */
+void ref_mat_inverse2d(const long double* a, long double* out);
+void ref_mat_mul(const long double* a, const long double* b, long double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
