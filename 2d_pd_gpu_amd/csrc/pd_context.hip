@@ -1,0 +1,1034 @@
+// pd_context.hip -- context, data layout and the C ABI of lib2dpdgpu.
+//
+// The caller's particles are reordered along a Morton curve at creation so that
+// a family's neighbours sit in nearby cache lines; bonds stay grouped per
+// family in the caller's within-family order.  All state lives in device
+// memory for the context's lifetime; host arrays cross the ABI only in the
+// caller's order (the permutation never leaks).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "pd_hip.h"
+#include "pd_internal.h"
+
+using namespace pd;
+
+static thread_local std::string g_err;
+
+struct pd_context {
+    pd_params p;
+    int n = 0;
+    long nb = 0;
+    int maxM = 0;
+    int lpf = 32;
+    int n2_is_10 = 1;
+    int dev = 0;
+    std::vector<int> perm;   // internal -> caller
+    std::vector<int> iperm;  // caller -> internal
+    std::vector<long> bmap;  // internal bond -> caller bond
+    std::vector<int> row_host, rev_host;  // internal CSR
+    std::vector<double> X_caller;         // 2n, caller order
+    std::vector<double> V_int;            // internal order
+    // device buffers
+    double2 *X = nullptr, *u = nullptr, *v = nullptr, *a = nullptr, *f = nullptr, *fown = nullptr;
+    double2* v0 = nullptr;
+    double *V = nullptr, *E = nullptr, *kinv = nullptr;
+    int *row = nullptr, *col = nullptr, *rev = nullptr, *bc = nullptr, *ids = nullptr;
+    long ids_cap = 0;
+    unsigned char* broken = nullptr;
+    void* W = nullptr;
+    unsigned int* counters = nullptr;
+    int n_groups = 0;
+    bool has_bc = false;
+    // execution
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int use_graph = 1, profile = 0, wpb = 4, xcd = 8, lpf_opt = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_valid = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> ev_pool;  // profiling: start/stop pairs
+    std::vector<int> ev_phase;
+    size_t ev_used = 0;
+    double phase_ms[4] = {0, 0, 0, 0};
+    long phase_calls[4] = {0, 0, 0, 0};
+    long steps = 0;
+    long n_singular = 0;
+    std::string err;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static int fail(pd_context* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                     \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((c), PD_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+template <typename F>
+static void parallel_for(long n, F fn)
+{
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        fn(0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    long chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        long b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([=]() { fn(b, e); });
+    }
+    for (auto& t : th) t.join();
+}
+
+static uint64_t spread32(uint64_t x)
+{
+    x &= 0xffffffffULL;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffULL;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffULL;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0fULL;
+    x = (x | (x << 2)) & 0x3333333333333333ULL;
+    x = (x | (x << 1)) & 0x5555555555555555ULL;
+    return x;
+}
+
+template <typename T>
+static int dalloc(pd_context* c, T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess)
+        return fail(c, PD_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", sizeof(T) * count,
+                    hipGetErrorString(e));
+    return PD_OK;
+}
+
+static void free_all(pd_context* c)
+{
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    void* ptrs[] = {c->X, c->u, c->v, c->a, c->f, c->fown, c->v0, c->V, c->E, c->kinv,
+                    c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->W, c->counters};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+}
+
+static BondArgs bond_args(pd_context* c)
+{
+    BondArgs a;
+    a.n = c->n;
+    a.xcd = c->xcd;
+    a.X = c->X;
+    a.V = c->V;
+    a.row = c->row;
+    a.col = c->col;
+    a.rev = c->rev;
+    a.u = c->u;
+    a.broken = c->broken;
+    a.kinv = c->kinv;
+    a.W = c->W;
+    a.fown = c->fown;
+    a.E = c->E;
+    a.E0 = c->p.E;
+    a.nu = c->p.nu;
+    a.A = c->p.n1 / (c->p.horizon * c->p.dx);
+    a.n2 = c->p.n2;
+    a.s0 = c->p.s0;
+    a.ktol = c->p.kappa_tol > 0.0 ? c->p.kappa_tol : 1e-10;
+    a.force_k = 0;
+    a.counters = c->counters;
+    return a;
+}
+
+static GatherArgs gather_args(pd_context* c, int write_a)
+{
+    GatherArgs g;
+    g.n = c->n;
+    g.xcd = c->xcd;
+    g.row = c->row;
+    g.fown = c->fown;
+    g.W = c->W;
+    g.v = c->v;
+    g.a = c->a;
+    g.f = c->f;
+    g.rho = c->p.rho;
+    g.dt = c->p.dt;
+    g.write_a = write_a;
+    return g;
+}
+
+static PosArgs pos_args(pd_context* c, bool with_bc)
+{
+    PosArgs q;
+    q.n = c->n;
+    q.u = c->u;
+    q.v = c->v;
+    q.a = c->a;
+    q.f = c->f;
+    q.bc = with_bc ? c->bc : nullptr;
+    q.v0 = c->v0;
+    q.rho = c->p.rho;
+    q.dt = c->p.dt;
+    return q;
+}
+
+// Resolve recorded event pairs into per-phase totals (profiling mode).
+static int drain_events(pd_context* c)
+{
+    for (size_t k = 0; k < c->ev_used; ++k) {
+        HIPCHK(c, hipEventSynchronize(c->ev_pool[2 * k + 1]));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[2 * k], c->ev_pool[2 * k + 1]));
+        c->phase_ms[c->ev_phase[k]] += ms;
+        c->phase_calls[c->ev_phase[k]] += 1;
+    }
+    c->ev_used = 0;
+    return PD_OK;
+}
+
+// Launch with optional per-phase timing (phase: 0 positions, 1 bonds, 2 gather,
+// 3 other).  Timing records a start/stop event pair on the launch stream around
+// each kernel without synchronising; pairs are resolved when the pool fills or
+// at the next synchronisation point.
+template <typename L>
+static int timed(pd_context* c, int phase, L launch)
+{
+    size_t slot = 0;
+    if (c->profile) {
+        if (c->ev_used == c->ev_phase.size()) {
+            if (c->ev_used >= 4096) {
+                if (int rc = drain_events(c)) return rc;
+            } else {
+                hipEvent_t a, b;
+                HIPCHK(c, hipEventCreate(&a));
+                HIPCHK(c, hipEventCreate(&b));
+                c->ev_pool.push_back(a);
+                c->ev_pool.push_back(b);
+                c->ev_phase.push_back(0);
+            }
+        }
+        slot = c->ev_used++;
+        c->ev_phase[slot] = phase;
+        HIPCHK(c, hipEventRecord(c->ev_pool[2 * slot], c->stream));
+    }
+    hipError_t e = launch();
+    if (e != hipSuccess) return fail(c, PD_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    if (c->profile) HIPCHK(c, hipEventRecord(c->ev_pool[2 * slot + 1], c->stream));
+    return PD_OK;
+}
+
+static int run_shape(pd_context* c)
+{
+    HIPCHK(c, launch_fill_u32(c->counters + 1, 0u, 1, c->stream));
+    BondArgs a = bond_args(c);
+    a.force_k = 1;
+    int rc = timed(c, 3, [&]() { return launch_shape(a, c->lpf, c->n2_is_10, c->wpb, c->stream); });
+    if (rc) return rc;
+    unsigned int h[2];
+    HIPCHK(c, hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->n_singular = h[1];
+    return PD_OK;
+}
+
+static int ensure_ids(pd_context* c, int count, const int* ids)
+{
+    if (count > c->ids_cap) {
+        if (c->ids) hipFree(c->ids);
+        c->ids = nullptr;
+        if (int rc = dalloc(c, &c->ids, (size_t)count)) return rc;
+        c->ids_cap = count;
+    }
+    std::vector<int> m(count);
+    for (int t = 0; t < count; ++t) {
+        if (ids[t] < 0 || ids[t] >= c->n) return fail(c, PD_ERR_INVALID, "particle id %d out of range", ids[t]);
+        m[t] = c->iperm[ids[t]];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->ids, m.data(), sizeof(int) * count, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// creation
+// ---------------------------------------------------------------------------
+static int check_params(const pd_params* p)
+{
+    if (!p) return fail(nullptr, PD_ERR_INVALID, "params is NULL");
+    if (p->ndim != 2) return fail(nullptr, PD_ERR_UNSUPPORTED, "only ndim == 2 is supported (got %d)", p->ndim);
+    if (p->precision != PD_FP64 && p->precision != PD_MIXED)
+        return fail(nullptr, PD_ERR_INVALID, "unknown precision %d", p->precision);
+    if (!(p->dx > 0) || !(p->horizon > 0) || !(p->rho > 0) || !(p->dt > 0) || !(p->E > 0))
+        return fail(nullptr, PD_ERR_INVALID, "dx, horizon, rho, dt and E must be positive");
+    if (!(p->nu > -1.0 && p->nu < 1.0)) return fail(nullptr, PD_ERR_INVALID, "nu must be in (-1, 1)");
+    if (!(p->n2 >= 0) || !(p->n1 >= 0)) return fail(nullptr, PD_ERR_INVALID, "n1, n2 must be >= 0");
+    return PD_OK;
+}
+
+extern "C" int pd_create(const pd_params* p, int n, const double* X, const double* V,
+                         const int* row_ptr, const int* col, pd_context** out)
+{
+    if (!out) return fail(nullptr, PD_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (int rc = check_params(p)) return rc;
+    if (n <= 0 || !X || !V || !row_ptr || (row_ptr[n] > 0 && !col))
+        return fail(nullptr, PD_ERR_INVALID, "empty mesh or NULL array");
+    if (row_ptr[0] != 0) return fail(nullptr, PD_ERR_INVALID, "row_ptr[0] must be 0");
+    const long nb = row_ptr[n];
+    int maxM = 0;
+    for (int i = 0; i < n; ++i) {
+        int m = row_ptr[i + 1] - row_ptr[i];
+        if (m < 0) return fail(nullptr, PD_ERR_INVALID, "row_ptr decreases at %d", i);
+        maxM = std::max(maxM, m);
+        if (!(V[i] > 0) || !std::isfinite(X[2 * i]) || !std::isfinite(X[2 * i + 1]))
+            return fail(nullptr, PD_ERR_INVALID, "particle %d: non-positive volume or non-finite position", i);
+    }
+    if (maxM > 64)
+        return fail(nullptr, PD_ERR_UNSUPPORTED, "family of %d bonds: at most 64 are supported", maxM);
+    std::atomic<long> bad{-1};
+    parallel_for(n, [&](long b0, long b1) {
+        for (long i = b0; i < b1; ++i)
+            for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+                if (col[e] < 0 || col[e] >= n || col[e] == i) bad = e;
+    });
+    if (bad >= 0) return fail(nullptr, PD_ERR_INVALID, "bond %ld: neighbour index out of range or self", (long)bad);
+
+    pd_context* c = new pd_context();
+    c->p = *p;
+    c->n = n;
+    c->nb = nb;
+    c->maxM = maxM;
+    c->lpf = maxM <= 32 ? 32 : 64;
+    c->n2_is_10 = (p->n2 == 10.0);
+
+    // --- Morton order on the dx grid ---
+    double xmin = X[0], ymin = X[1];
+    for (int i = 0; i < n; ++i) {
+        xmin = std::min(xmin, X[2 * i]);
+        ymin = std::min(ymin, X[2 * i + 1]);
+    }
+    std::vector<uint64_t> key(n);
+    parallel_for(n, [&](long b0, long b1) {
+        for (long i = b0; i < b1; ++i) {
+            double qx = std::floor((X[2 * i] - xmin) / p->dx + 0.5);
+            double qy = std::floor((X[2 * i + 1] - ymin) / p->dx + 0.5);
+            uint64_t ix = (uint64_t)std::min(std::max(qx, 0.0), 4294967295.0);
+            uint64_t iy = (uint64_t)std::min(std::max(qy, 0.0), 4294967295.0);
+            key[i] = spread32(ix) | (spread32(iy) << 1);
+        }
+    });
+    c->perm.resize(n);
+    std::iota(c->perm.begin(), c->perm.end(), 0);
+    std::stable_sort(c->perm.begin(), c->perm.end(), [&](int x, int y) { return key[x] < key[y]; });
+    c->iperm.resize(n);
+    for (int q = 0; q < n; ++q) c->iperm[c->perm[q]] = q;
+
+    // --- internal CSR ---
+    std::vector<int> row(n + 1), coli(nb > 0 ? nb : 1);
+    c->bmap.resize(nb);
+    row[0] = 0;
+    for (int q = 0; q < n; ++q) {
+        int i = c->perm[q];
+        row[q + 1] = row[q] + (row_ptr[i + 1] - row_ptr[i]);
+    }
+    parallel_for(n, [&](long q0, long q1) {
+        for (long q = q0; q < q1; ++q) {
+            int i = c->perm[q];
+            long bi = row[q];
+            for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e, ++bi) {
+                coli[bi] = c->iperm[col[e]];
+                c->bmap[bi] = e;
+            }
+        }
+    });
+    // --- reverse bonds: (q -> r) finds q in row r ---
+    std::vector<int> rev(nb > 0 ? nb : 1);
+    std::atomic<long> asym{-1};
+    parallel_for(n, [&](long q0, long q1) {
+        for (long q = q0; q < q1; ++q) {
+            for (int b = row[q]; b < row[q + 1]; ++b) {
+                int r = coli[b];
+                int found = -1;
+                for (int e = row[r]; e < row[r + 1]; ++e)
+                    if (coli[e] == q) { found = e; break; }
+                if (found < 0) asym = b;
+                rev[b] = found;
+            }
+        }
+    });
+    if (asym >= 0) {
+        long b = asym;
+        int rc = fail(nullptr, PD_ERR_INVALID, "families are not symmetric: caller bond %ld has no reverse", c->bmap[b]);
+        delete c;
+        return rc;
+    }
+    c->row_host = row;
+    c->rev_host = rev;
+    c->X_caller.assign(X, X + 2 * (size_t)n);
+
+    std::vector<double2> Xi(n);
+    c->V_int.resize(n);
+    for (int q = 0; q < n; ++q) {
+        int i = c->perm[q];
+        Xi[q] = make_double2(X[2 * i], X[2 * i + 1]);
+        c->V_int[q] = V[i];
+    }
+
+    // --- device buffers ---
+    if (hipGetDevice(&c->dev) != hipSuccess) {
+        delete c;
+        return fail(nullptr, PD_ERR_HIP, "no HIP device available");
+    }
+    int rc = PD_OK;
+    const size_t wbytes = c->p.precision == PD_MIXED ? 8 : 16;
+    if ((rc = dalloc(c, &c->X, n)) || (rc = dalloc(c, &c->u, n)) || (rc = dalloc(c, &c->v, n)) ||
+        (rc = dalloc(c, &c->a, n)) || (rc = dalloc(c, &c->f, n)) || (rc = dalloc(c, &c->fown, n)) ||
+        (rc = dalloc(c, &c->V, n)) || (rc = dalloc(c, &c->row, n + 1)) ||
+        (rc = dalloc(c, &c->col, nb)) || (rc = dalloc(c, &c->rev, nb)) ||
+        (rc = dalloc(c, &c->kinv, 3 * (size_t)nb)) || (rc = dalloc(c, &c->broken, nb)) ||
+        (rc = dalloc(c, (char**)&c->W, wbytes * (size_t)(nb > 0 ? nb : 1))) ||
+        (rc = dalloc(c, &c->counters, 4)) || (rc = dalloc(c, &c->bc, n)) ||
+        (rc = dalloc(c, &c->v0, 1))) {
+        free_all(c);
+        delete c;
+        return rc;
+    }
+    auto bail = [&](int code) { free_all(c); delete c; return code; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, PD_ERR_HIP, "hipStreamCreate failed"));
+    c->own_stream = true;
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(nullptr, PD_ERR_HIP, "hipEventCreate failed"));
+    hipError_t e = hipSuccess;
+    e = e != hipSuccess ? e : hipMemcpy(c->X, Xi.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
+    e = e != hipSuccess ? e : hipMemcpy(c->V, c->V_int.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    e = e != hipSuccess ? e : hipMemcpy(c->row, row.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice);
+    if (nb > 0) {
+        e = e != hipSuccess ? e : hipMemcpy(c->col, coli.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
+        e = e != hipSuccess ? e : hipMemcpy(c->rev, rev.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
+        e = e != hipSuccess ? e : hipMemset(c->broken, 0, nb);
+        e = e != hipSuccess ? e : hipMemset(c->W, 0, wbytes * nb);
+    }
+    e = e != hipSuccess ? e : hipMemset(c->u, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->v, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->a, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->f, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->fown, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->counters, 0, 16);
+    e = e != hipSuccess ? e : hipDeviceSynchronize();
+    if (e != hipSuccess) return bail(fail(nullptr, PD_ERR_HIP, "upload failed: %s", hipGetErrorString(e)));
+    {
+        std::vector<int> code(n, PD_BC_FREE);
+        if (hipMemcpy(c->bc, code.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(nullptr, PD_ERR_HIP, "upload failed"));
+    }
+    if ((rc = run_shape(c))) {
+        g_err = c->err;
+        return bail(rc);
+    }
+    *out = c;
+    return PD_OK;
+}
+
+extern "C" int pd_create_padded(const pd_params* p, int n, const double* X, const double* V,
+                                const int* nbr, const int* nbr_count, int cap, pd_context** out)
+{
+    if (n <= 0 || !nbr || !nbr_count || cap <= 0) return fail(nullptr, PD_ERR_INVALID, "bad padded layout");
+    std::vector<int> row(n + 1, 0), col;
+    for (int i = 0; i < n; ++i) {
+        int m = nbr_count[i];
+        if (m < 0 || m > cap) return fail(nullptr, PD_ERR_INVALID, "nbr_count[%d] = %d outside [0, %d]", i, m, cap);
+        for (int t = 0; t < m; ++t) {
+            int j = nbr[(size_t)i * cap + t];
+            if (j == INT_MAX) return fail(nullptr, PD_ERR_INVALID, "padding inside family %d", i);
+            col.push_back(j);
+        }
+        row[i + 1] = (int)col.size();
+    }
+    return pd_create(p, n, X, V, row.data(), col.data(), out);
+}
+
+extern "C" void pd_destroy(pd_context* c)
+{
+    if (!c) return;
+    hipSetDevice(c->dev);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+}
+
+extern "C" const char* pd_last_error(const pd_context* c)
+{
+    if (c) return c->err.c_str();
+    return g_err.c_str();
+}
+
+extern "C" const char* pd_version(void) { return "lib2dpdgpu 0.1 gfx950"; }
+
+extern "C" int pd_set_stream(pd_context* c, void* s)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    hipSetDevice(c->dev);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (s) {
+        if (c->own_stream) hipStreamDestroy(c->stream);
+        c->stream = (hipStream_t)s;
+        c->own_stream = false;
+    } else if (!c->own_stream) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    c->graph_valid = false;
+    return PD_OK;
+}
+
+extern "C" int pd_synchronize(pd_context* c)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PD_OK;
+}
+
+extern "C" int pd_set_option(pd_context* c, int opt, long value)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    switch (opt) {
+    case PD_OPT_USE_GRAPH: c->use_graph = value != 0; break;
+    case PD_OPT_PROFILE: c->profile = value != 0; break;
+    case PD_OPT_WAVES_PER_WG:
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+            return fail(c, PD_ERR_INVALID, "waves per workgroup must be 1, 2, 4 or 8");
+        c->wpb = (int)value;
+        break;
+    case PD_OPT_XCD_REMAP: c->xcd = value ? 8 : 1; break;
+    case PD_OPT_LANES_PER_FAM:
+        if (value == 0) c->lpf = c->maxM <= 32 ? 32 : 64;
+        else if (value == 64 || (value == 32 && c->maxM <= 32)) c->lpf = (int)value;
+        else return fail(c, PD_ERR_INVALID, "lanes per family must be 64, or 32 when families have <= 32 bonds");
+        break;
+    default: return fail(c, PD_ERR_INVALID, "unknown option %d", opt);
+    }
+    c->graph_valid = false;
+    return PD_OK;
+}
+
+extern "C" int pd_set_youngs_modulus(pd_context* c, const double* E)
+{
+    if (!c || !E) return fail(c, PD_ERR_INVALID, "NULL argument");
+    std::vector<double> Ei(c->n);
+    for (int q = 0; q < c->n; ++q) {
+        Ei[q] = E[c->perm[q]];
+        if (!(Ei[q] > 0)) return fail(c, PD_ERR_INVALID, "E[%d] must be positive", c->perm[q]);
+    }
+    if (!c->E)
+        if (int rc = dalloc(c, &c->E, c->n)) return rc;
+    HIPCHK(c, hipMemcpy(c->E, Ei.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
+    c->graph_valid = false;
+    return PD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// state
+// ---------------------------------------------------------------------------
+static int put_vec(pd_context* c, double2* dst, const double* src)
+{
+    if (!src) return PD_OK;
+    std::vector<double2> t(c->n);
+    for (int q = 0; q < c->n; ++q) {
+        int i = c->perm[q];
+        t[q] = make_double2(src[2 * i], src[2 * i + 1]);
+    }
+    HIPCHK(c, hipMemcpyAsync(dst, t.data(), sizeof(double2) * c->n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PD_OK;
+}
+
+static int get_vec(pd_context* c, const double2* src, double* dst)
+{
+    if (!dst) return PD_OK;
+    std::vector<double2> t(c->n);
+    HIPCHK(c, hipMemcpyAsync(t.data(), src, sizeof(double2) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int q = 0; q < c->n; ++q) {
+        int i = c->perm[q];
+        dst[2 * i] = t[q].x;
+        dst[2 * i + 1] = t[q].y;
+    }
+    return PD_OK;
+}
+
+extern "C" int pd_set_state(pd_context* c, const double* u, const double* v, const double* a,
+                            const double* f)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    hipSetDevice(c->dev);
+    int rc;
+    if ((rc = put_vec(c, c->u, u)) || (rc = put_vec(c, c->v, v)) || (rc = put_vec(c, c->a, a)) ||
+        (rc = put_vec(c, c->f, f)))
+        return rc;
+    return PD_OK;
+}
+
+extern "C" int pd_get_state(pd_context* c, double* u, double* v, double* a, double* f)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    hipSetDevice(c->dev);
+    int rc;
+    if ((rc = get_vec(c, c->u, u)) || (rc = get_vec(c, c->v, v)) || (rc = get_vec(c, c->a, a)) ||
+        (rc = get_vec(c, c->f, f)))
+        return rc;
+    return PD_OK;
+}
+
+extern "C" int pd_set_positions(pd_context* c, const double* y)
+{
+    if (!c || !y) return fail(c, PD_ERR_INVALID, "NULL argument");
+    std::vector<double> u(2 * (size_t)c->n);
+    for (size_t t = 0; t < u.size(); ++t) u[t] = y[t] - c->X_caller[t];
+    return pd_set_state(c, u.data(), nullptr, nullptr, nullptr);
+}
+
+extern "C" int pd_get_positions(pd_context* c, double* y)
+{
+    if (!c || !y) return fail(c, PD_ERR_INVALID, "NULL argument");
+    if (int rc = pd_get_state(c, y, nullptr, nullptr, nullptr)) return rc;
+    for (size_t t = 0; t < 2 * (size_t)c->n; ++t) y[t] += c->X_caller[t];
+    return PD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phase calls
+// ---------------------------------------------------------------------------
+extern "C" int pd_apply_velocity_bc(pd_context* c, int count, const int* ids, const double* v0)
+{
+    if (!c || (count > 0 && (!ids || !v0))) return fail(c, PD_ERR_INVALID, "NULL argument");
+    if (count <= 0) return PD_OK;
+    hipSetDevice(c->dev);
+    if (int rc = ensure_ids(c, count, ids)) return rc;
+    return timed(c, 3, [&]() {
+        return launch_velocity_bc_list(count, c->ids, make_double2(v0[0], v0[1]), c->v, c->a, c->stream);
+    });
+}
+
+extern "C" int pd_apply_fixed_bc(pd_context* c, int count, const int* ids)
+{
+    if (!c || (count > 0 && !ids)) return fail(c, PD_ERR_INVALID, "NULL argument");
+    if (count <= 0) return PD_OK;
+    hipSetDevice(c->dev);
+    if (int rc = ensure_ids(c, count, ids)) return rc;
+    return timed(c, 3, [&]() { return launch_fixed_bc_list(count, c->ids, c->u, c->stream); });
+}
+
+extern "C" int pd_update_positions(pd_context* c)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    hipSetDevice(c->dev);
+    PosArgs q = pos_args(c, false);
+    return timed(c, 0, [&]() { return launch_positions(q, c->stream); });
+}
+
+static int force_and_velocity(pd_context* c, int write_a)
+{
+    BondArgs a = bond_args(c);
+    GatherArgs g = gather_args(c, write_a);
+    int mixed = c->p.precision == PD_MIXED;
+    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->n2_is_10, c->wpb, c->stream); }))
+        return rc;
+    return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream); });
+}
+
+extern "C" int pd_compute_velocity(pd_context* c, int write_acceleration)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    hipSetDevice(c->dev);
+    if (int rc = force_and_velocity(c, write_acceleration)) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PD_OK;
+}
+
+extern "C" int pd_set_boundary(pd_context* c, const int* code, int n_groups, const double* v0)
+{
+    if (!c || !code || n_groups < 0 || (n_groups > 0 && !v0)) return fail(c, PD_ERR_INVALID, "bad boundary program");
+    hipSetDevice(c->dev);
+    std::vector<int> ci(c->n);
+    for (int q = 0; q < c->n; ++q) {
+        int g = code[c->perm[q]];
+        if (g < PD_BC_FIXED || g >= n_groups)
+            return fail(c, PD_ERR_INVALID, "boundary code %d of particle %d outside [-2, %d)", g, c->perm[q], n_groups);
+        ci[q] = g;
+    }
+    if (n_groups > c->n_groups) {
+        hipFree(c->v0);
+        c->v0 = nullptr;
+        if (int rc = dalloc(c, &c->v0, (size_t)n_groups)) return rc;
+    }
+    c->n_groups = std::max(c->n_groups, n_groups);
+    HIPCHK(c, hipMemcpy(c->bc, ci.data(), sizeof(int) * c->n, hipMemcpyHostToDevice));
+    if (n_groups > 0) HIPCHK(c, hipMemcpy(c->v0, v0, sizeof(double2) * n_groups, hipMemcpyHostToDevice));
+    c->has_bc = true;
+    c->graph_valid = false;
+    return PD_OK;
+}
+
+static int enqueue_step(pd_context* c)
+{
+    PosArgs q = pos_args(c, true);
+    if (int rc = timed(c, 0, [&]() { return launch_positions(q, c->stream); })) return rc;
+    return force_and_velocity(c, 0);
+}
+
+extern "C" int pd_step(pd_context* c, int nsteps)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    if (nsteps < 0) return fail(c, PD_ERR_INVALID, "nsteps < 0");
+    hipSetDevice(c->dev);
+    if (c->use_graph && !c->profile && nsteps > 1) {
+        if (!c->graph_valid) {
+            if (c->gexec) hipGraphExecDestroy(c->gexec);
+            if (c->graph) hipGraphDestroy(c->graph);
+            c->gexec = nullptr;
+            c->graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_step(c);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(c, PD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            c->graph = g;
+            HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+            c->graph_valid = true;
+        }
+        for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    } else {
+        for (int s = 0; s < nsteps; ++s)
+            if (int rc = enqueue_step(c)) return rc;
+    }
+    c->steps += nsteps;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// bonds and diagnostics
+// ---------------------------------------------------------------------------
+extern "C" int pd_get_broken(pd_context* c, unsigned char* mask)
+{
+    if (!c || !mask) return fail(c, PD_ERR_INVALID, "NULL argument");
+    hipSetDevice(c->dev);
+    std::vector<unsigned char> t(c->nb);
+    if (c->nb) {
+        HIPCHK(c, hipMemcpyAsync(t.data(), c->broken, c->nb, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    for (long b = 0; b < c->nb; ++b) mask[c->bmap[b]] = t[b];
+    return PD_OK;
+}
+
+extern "C" int pd_set_broken(pd_context* c, const unsigned char* mask)
+{
+    if (!c || !mask) return fail(c, PD_ERR_INVALID, "NULL argument");
+    hipSetDevice(c->dev);
+    std::vector<unsigned char> t(c->nb);
+    for (long b = 0; b < c->nb; ++b) t[b] = mask[c->bmap[b]] ? 1 : 0;
+    for (long b = 0; b < c->nb; ++b)
+        if (t[b] != t[c->rev_host[b]])
+            return fail(c, PD_ERR_INVALID, "broken mask must be symmetric (caller bond %ld)", c->bmap[b]);
+    if (c->nb) HIPCHK(c, hipMemcpy(c->broken, t.data(), c->nb, hipMemcpyHostToDevice));
+    return run_shape(c);
+}
+
+extern "C" int pd_get_bond_forces(pd_context* c, double* T)
+{
+    if (!c || !T) return fail(c, PD_ERR_INVALID, "NULL argument");
+    hipSetDevice(c->dev);
+    std::vector<double> w(2 * (size_t)c->nb);
+    if (c->nb) {
+        if (c->p.precision == PD_MIXED) {
+            std::vector<float> wf(2 * (size_t)c->nb);
+            HIPCHK(c, hipMemcpy(wf.data(), c->W, sizeof(float) * wf.size(), hipMemcpyDeviceToHost));
+            for (size_t t = 0; t < wf.size(); ++t) w[t] = wf[t];
+        } else {
+            HIPCHK(c, hipMemcpy(w.data(), c->W, sizeof(double) * w.size(), hipMemcpyDeviceToHost));
+        }
+    }
+    // slot rev[b] holds T_ij V_i for b = (i -> j)
+    for (int q = 0; q < c->n; ++q) {
+        double Vi = c->V_int[q];
+        for (int b = c->row_host[q]; b < c->row_host[q + 1]; ++b) {
+            long r = c->rev_host[b];
+            long e = c->bmap[b];
+            T[2 * e] = w[2 * r] / Vi;
+            T[2 * e + 1] = w[2 * r + 1] / Vi;
+        }
+    }
+    return PD_OK;
+}
+
+extern "C" int pd_get_stats(pd_context* c, pd_stats* s)
+{
+    if (!c || !s) return fail(c, PD_ERR_INVALID, "NULL argument");
+    hipSetDevice(c->dev);
+    if (int rc = drain_events(c)) return rc;
+    memset(s, 0, sizeof(*s));
+    s->n = c->n;
+    s->nb = c->nb;
+    s->max_family = c->maxM;
+    s->lanes_per_family = c->lpf;
+    s->steps = c->steps;
+    s->n_singular = c->n_singular;
+    std::vector<unsigned char> t(c->nb);
+    if (c->nb) {
+        HIPCHK(c, hipMemcpy(t.data(), c->broken, c->nb, hipMemcpyDeviceToHost));
+        long nbk = 0;
+        for (unsigned char x : t) nbk += x;
+        s->n_broken = nbk;
+    }
+    for (int k = 0; k < 4; ++k) {
+        s->phase_ms[k] = c->phase_ms[k];
+        s->phase_calls[k] = c->phase_calls[k];
+    }
+    return PD_OK;
+}
+
+extern "C" void pd_reset_profile(pd_context* c)
+{
+    if (!c) return;
+    drain_events(c);
+    for (int k = 0; k < 4; ++k) {
+        c->phase_ms[k] = 0;
+        c->phase_calls[k] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: launch-shape variants timed on the context's own state
+// ---------------------------------------------------------------------------
+extern "C" int pd_autotune(pd_context* c, int steps)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    if (steps <= 0) steps = 5;
+    hipSetDevice(c->dev);
+    const size_t n = c->n, nb = c->nb;
+    double2 *su = nullptr, *sv = nullptr, *sa = nullptr, *sf = nullptr;
+    unsigned char* sb = nullptr;
+    double* sk = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &su, n)) || (rc = dalloc(c, &sv, n)) || (rc = dalloc(c, &sa, n)) ||
+        (rc = dalloc(c, &sf, n)) || (rc = dalloc(c, &sb, nb)) || (rc = dalloc(c, &sk, 3 * nb)))
+        return rc;
+    auto save = [&](bool restore) -> hipError_t {
+        hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        hipError_t e = hipSuccess;
+        if (!restore) {
+            e = e ? e : hipMemcpyAsync(su, c->u, 16 * n, k, c->stream);
+            e = e ? e : hipMemcpyAsync(sv, c->v, 16 * n, k, c->stream);
+            e = e ? e : hipMemcpyAsync(sa, c->a, 16 * n, k, c->stream);
+            e = e ? e : hipMemcpyAsync(sf, c->f, 16 * n, k, c->stream);
+            if (nb) e = e ? e : hipMemcpyAsync(sb, c->broken, nb, k, c->stream);
+            if (nb) e = e ? e : hipMemcpyAsync(sk, c->kinv, 24 * nb, k, c->stream);
+        } else {
+            e = e ? e : hipMemcpyAsync(c->u, su, 16 * n, k, c->stream);
+            e = e ? e : hipMemcpyAsync(c->v, sv, 16 * n, k, c->stream);
+            e = e ? e : hipMemcpyAsync(c->a, sa, 16 * n, k, c->stream);
+            e = e ? e : hipMemcpyAsync(c->f, sf, 16 * n, k, c->stream);
+            if (nb) e = e ? e : hipMemcpyAsync(c->broken, sb, nb, k, c->stream);
+            if (nb) e = e ? e : hipMemcpyAsync(c->kinv, sk, 24 * nb, k, c->stream);
+        }
+        return e ? e : hipStreamSynchronize(c->stream);
+    };
+    auto cleanup = [&]() {
+        hipFree(su); hipFree(sv); hipFree(sa); hipFree(sf); hipFree(sb); hipFree(sk);
+    };
+    if (save(false) != hipSuccess) {
+        cleanup();
+        return fail(c, PD_ERR_HIP, "autotune: state snapshot failed");
+    }
+    const int saved_profile = c->profile;
+    const long saved_steps = c->steps;
+    c->profile = 0;
+    struct Var { int wpb, xcd, lpf; };
+    std::vector<Var> vars;
+    for (int lpf : {32, 64}) {
+        if (lpf == 32 && c->maxM > 32) continue;
+        for (int w : {1, 2, 4, 8})
+            for (int x : {1, 8}) vars.push_back({w, x, lpf});
+    }
+    double best = 1e300;
+    Var bestv = {c->wpb, c->xcd, c->lpf};
+    for (const Var& vv : vars) {
+        c->wpb = vv.wpb;
+        c->xcd = vv.xcd;
+        c->lpf = vv.lpf;
+        c->graph_valid = false;
+        if ((rc = pd_step(c, 1))) break;  // warm-up
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        if ((rc = pd_step(c, steps))) break;
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (ms < best) {
+            best = ms;
+            bestv = vv;
+        }
+        if (save(true) != hipSuccess) {
+            rc = fail(c, PD_ERR_HIP, "autotune: state restore failed");
+            break;
+        }
+    }
+    c->wpb = bestv.wpb;
+    c->xcd = bestv.xcd;
+    c->lpf = bestv.lpf;
+    c->graph_valid = false;
+    c->profile = saved_profile;
+    c->steps = saved_steps;
+    cleanup();
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// setup helper: families from a cell list (findNeighbor semantics)
+// ---------------------------------------------------------------------------
+extern "C" long pd_build_families(int n, const double* X, double dx, double horizon, int* row_ptr,
+                                  int* col, long cap)
+{
+    if (n <= 0 || !X || !row_ptr || !(dx > 0) || !(horizon > 0)) return -1;
+    const long double cut = (long double)1.01 * (long double)dx * (long double)horizon;
+    const double cs = (double)cut;
+    double xmin = X[0], ymin = X[1], xmax = X[0], ymax = X[1];
+    for (int i = 0; i < n; ++i) {
+        xmin = std::min(xmin, X[2 * i]);
+        xmax = std::max(xmax, X[2 * i]);
+        ymin = std::min(ymin, X[2 * i + 1]);
+        ymax = std::max(ymax, X[2 * i + 1]);
+    }
+    const long ncx = std::max(1L, (long)std::floor((xmax - xmin) / cs) + 1);
+    const long ncy = std::max(1L, (long)std::floor((ymax - ymin) / cs) + 1);
+    std::vector<long> cstart(ncx * ncy + 1, 0);
+    std::vector<int> cell(n), order(n);
+    for (int i = 0; i < n; ++i) {
+        long cx = std::min(ncx - 1, (long)std::floor((X[2 * i] - xmin) / cs));
+        long cy = std::min(ncy - 1, (long)std::floor((X[2 * i + 1] - ymin) / cs));
+        cell[i] = (int)(cy * ncx + cx);
+        cstart[cell[i] + 1]++;
+    }
+    for (long k = 0; k < ncx * ncy; ++k) cstart[k + 1] += cstart[k];
+    {
+        std::vector<long> fill(cstart.begin(), cstart.end() - 1);
+        for (int i = 0; i < n; ++i) order[fill[cell[i]]++] = i;
+    }
+    auto neighbours = [&](int i, std::vector<int>& out) {
+        out.clear();
+        long cx = cell[i] % ncx, cy = cell[i] / ncx;
+        for (long yy = std::max(0L, cy - 1); yy <= std::min(ncy - 1, cy + 1); ++yy)
+            for (long xx = std::max(0L, cx - 1); xx <= std::min(ncx - 1, cx + 1); ++xx) {
+                long k = yy * ncx + xx;
+                for (long t = cstart[k]; t < cstart[k + 1]; ++t) {
+                    int j = order[t];
+                    if (j == i) continue;
+                    long double r0 = (long double)X[2 * j] - (long double)X[2 * i];
+                    long double r1 = (long double)X[2 * j + 1] - (long double)X[2 * i + 1];
+                    long double d2 = 0.0L;
+                    d2 += r0 * r0;
+                    d2 += r1 * r1;
+                    if (sqrtl(d2) < cut) out.push_back(j);
+                }
+            }
+        std::sort(out.begin(), out.end());
+    };
+    std::vector<int> cnt(n);
+    parallel_for(n, [&](long b0, long b1) {
+        std::vector<int> tmp;
+        for (long i = b0; i < b1; ++i) {
+            neighbours((int)i, tmp);
+            cnt[i] = (int)tmp.size();
+        }
+    });
+    long total = 0;
+    row_ptr[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        total += cnt[i];
+        if (total > INT_MAX) return -1;
+        row_ptr[i + 1] = (int)total;
+    }
+    if (!col) return total;
+    if (total > cap) return -1;
+    parallel_for(n, [&](long b0, long b1) {
+        std::vector<int> tmp;
+        for (long i = b0; i < b1; ++i) {
+            neighbours((int)i, tmp);
+            std::copy(tmp.begin(), tmp.end(), col + row_ptr[i]);
+        }
+    });
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// setup helper: bonds crossing notch segments
+// ---------------------------------------------------------------------------
+static inline double orient(double ax, double ay, double bx, double by, double px, double py)
+{
+    return (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+}
+
+extern "C" long pd_notch_mask(int n, const double* X, const int* row_ptr, const int* col, int nseg,
+                              const double* seg, unsigned char* mask)
+{
+    if (n <= 0 || !X || !row_ptr || !mask || nseg < 0 || (nseg > 0 && !seg)) return -1;
+    std::atomic<long> total{0};
+    parallel_for(n, [&](long b0, long b1) {
+        long cnt = 0;
+        for (long i = b0; i < b1; ++i) {
+            const double px = X[2 * i], py = X[2 * i + 1];
+            for (int b = row_ptr[i]; b < row_ptr[i + 1]; ++b) {
+                const int j = col[b];
+                const double qx = X[2 * j], qy = X[2 * j + 1];
+                unsigned char hit = 0;
+                for (int s = 0; s < nseg && !hit; ++s) {
+                    const double ax = seg[4 * s], ay = seg[4 * s + 1], bx = seg[4 * s + 2], by = seg[4 * s + 3];
+                    const double d1 = orient(ax, ay, bx, by, px, py), d2 = orient(ax, ay, bx, by, qx, qy);
+                    const double d3 = orient(px, py, qx, qy, ax, ay), d4 = orient(px, py, qx, qy, bx, by);
+                    if (((d1 > 0 && d2 < 0) || (d1 < 0 && d2 > 0)) && ((d3 > 0 && d4 < 0) || (d3 < 0 && d4 > 0)))
+                        hit = 1;
+                }
+                mask[b] = hit;
+                cnt += hit;
+            }
+        }
+        total += cnt;
+    });
+    return total;
+}

@@ -1,0 +1,70 @@
+// pd_internal.h -- kernel argument blocks and launchers shared by the HIP
+// kernels (pd_kernels.hip) and the context / C-ABI layer (pd_context.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pd {
+
+// Everything the bond kernel needs for one launch.  Particle arrays are in the
+// internal (space-filling-curve) order; bond arrays in the internal CSR order,
+// one family (the bonds of particle i) contiguous.
+struct BondArgs {
+    int n;                    // families (= particles)
+    int xcd;                  // XCDs to spread consecutive blocks over (<= 1: off)
+    const double2* X;         // initial positions
+    const double* V;          // volumes
+    const int* row;           // CSR row pointer, n+1
+    const int* col;           // neighbour of each bond
+    const int* rev;           // reverse bond of each bond
+    const double2* u;         // displacement y - X
+    unsigned char* broken;    // per bond, 1 = broken (irreversible)
+    double* kinv;             // per bond K_ik^-1 (xx, xy, yy), FP64
+    void* W;                  // per bond, slot rev[b] <- T_ij V_i (double2 / float2)
+    double2* fown;            // per particle sum_j T_ij V_j
+    const double* E;          // per particle Young's modulus or nullptr
+    double E0, nu;            // uniform E, Poisson ratio
+    double A;                 // n1 / (horizon * dx)
+    double n2;                // angular exponent
+    double s0;                // critical stretch (<= 0: none)
+    double ktol;              // singular-K tolerance
+    int force_k;              // 1: assemble K for every family in this launch
+    unsigned int* counters;   // [0] newly broken bonds, [1] singular K
+};
+
+struct GatherArgs {
+    int n;
+    int xcd;
+    const int* row;
+    const double2* fown;
+    const void* W;
+    double2* v;
+    double2* a;
+    double2* f;
+    double rho, dt;
+    int write_a;
+};
+
+struct PosArgs {
+    int n;
+    double2* u;
+    double2* v;
+    double2* a;
+    const double2* f;
+    const int* bc;            // per particle code, or nullptr
+    const double2* v0;        // per velocity group
+    double rho, dt;
+};
+
+hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int n2_is_10, int wpb,
+                       hipStream_t s);
+hipError_t launch_shape(const BondArgs& a, int lpf, int n2_is_10, int wpb, hipStream_t s);
+hipError_t launch_gather(const GatherArgs& a, int mixed, hipStream_t s);
+hipError_t launch_positions(const PosArgs& a, hipStream_t s);
+hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double2* v,
+                                   double2* a, hipStream_t s);
+hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
+hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
+
+}  // namespace pd

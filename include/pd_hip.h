@@ -1,0 +1,183 @@
+/*
+ * pd_hip.h -- C ABI of lib2dpdgpu, the MI355X (gfx950) implementation of the
+ * explicit bond-associated NOSB peridynamics timestep of EricYJA/2D_PD_GPU.
+ *
+ * Plain C, POD arguments, status-code returns, no exceptions across the ABI.
+ * All arrays are host arrays in the CALLER's particle order and the caller's
+ * bond (CSR) order; the library reorders particles along a space-filling curve
+ * internally and keeps every buffer resident in device memory for the life of
+ * the context.  One context per host thread.
+ *
+ * Reference entry points replaced (EricYJA/2D_PD_GPU):
+ *   computeVelocity        src/main_seq.cpp:396-470, include/mechanics.h:21-23,
+ *                          compute_velocity_GPU_host include/mechanics.cuh:867-1136
+ *                            -> pd_compute_velocity
+ *   updatePositions        src/main_seq.cpp:386-393, include/mechanics.h:20
+ *                            -> pd_update_positions
+ *   applyVelocityBC        src/main_seq.cpp:28-36,  include/mechanics.h:9
+ *                            -> pd_apply_velocity_bc
+ *   applyFixedBC           src/main_seq.cpp:38-46,  include/mechanics.h:10
+ *                            -> pd_apply_fixed_bc
+ *   getStiffnessTensor     src/main_seq.cpp:180-201 -> pd_params {E, nu}
+ *   buildLocalNeighborlist src/main_seq.cpp:49-68, findNeighbor
+ *                          src/SetupParticleSystem.cpp:100-136
+ *                            -> pd_create (CSR) / pd_create_padded (the GPU
+ *                               path's padded layout, include/mechanics.cuh:
+ *                               1033-1046) / pd_build_families
+ *   step loop              src/main_seq.cpp:680-715 -> pd_step
+ * A header-only C++ adapter keeping the reference signatures is in
+ * include/pd_mechanics_adapter.hpp.
+ */
+#ifndef PD_HIP_H
+#define PD_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define PD_OK               0
+#define PD_ERR_INVALID      1  /* bad argument or inconsistent mesh            */
+#define PD_ERR_HIP          2  /* a HIP runtime call failed                    */
+#define PD_ERR_UNSUPPORTED  3  /* e.g. family larger than 64 bonds, ndim != 2  */
+#define PD_ERR_NOMEM        4  /* device or host allocation failed             */
+#define PD_ERR_NOT_FINITE   5  /* a non-finite value appeared in the state     */
+
+/* ---- precision ------------------------------------------------------------- */
+#define PD_FP64   0  /* every bond quantity in FP64                             */
+#define PD_MIXED  1  /* FP32 bond arithmetic (F, sigma, T, force state);
+                        FP64 K assembly/inversion and FP64 u, v, a, f           */
+
+/* ---- boundary codes (pd_set_boundary) --------------------------------------- */
+#define PD_BC_FREE  (-1)  /* no constraint                                       */
+#define PD_BC_FIXED (-2)  /* applyFixedBC: u = 0 (y = X) after the position update */
+                          /* g >= 0: applyVelocityBC with velocity group g       */
+
+/* ---- options (pd_set_option) ----------------------------------------------- */
+#define PD_OPT_USE_GRAPH     1  /* 1: pd_step replays a captured hipGraph (default 1) */
+#define PD_OPT_PROFILE       2  /* 1: time each kernel phase with HIP events          */
+#define PD_OPT_WAVES_PER_WG  3  /* families-per-wavefront tile: waves per workgroup   */
+#define PD_OPT_XCD_REMAP     4  /* 1: map consecutive families to one XCD (default 1) */
+#define PD_OPT_LANES_PER_FAM 5  /* 32 or 64 lanes per family (0 = by max family size)  */
+
+typedef struct pd_params {
+    int    ndim;       /* must be 2 */
+    int    precision;  /* PD_FP64 or PD_MIXED */
+    double n1, n2;     /* weight exponents (reference: 10, 10) */
+    double horizon;    /* delta in units of dx */
+    double dx;         /* grid spacing */
+    double rho;        /* mass density */
+    double dt;         /* step size */
+    double E;          /* Young's modulus (per-particle values: pd_set_youngs_modulus) */
+    double nu;         /* Poisson ratio (plane stress) */
+    double s0;         /* critical stretch; <= 0 disables bond breaking */
+    double kappa_tol;  /* K_ik treated as singular when det <= kappa_tol*tr^2; 0 -> 1e-10 */
+} pd_params;
+
+typedef struct pd_stats {
+    int    n;              /* particles */
+    long   nb;             /* directed bonds */
+    long   n_broken;       /* directed bonds currently broken */
+    long   n_singular;     /* bonds whose K was singular at the last assembly */
+    int    max_family;     /* largest family */
+    int    lanes_per_family;
+    long   steps;          /* steps run by pd_step since creation */
+    double phase_ms[4];    /* profiled time: positions, bonds, gather, other */
+    long   phase_calls[4];
+} pd_stats;
+
+typedef struct pd_context pd_context;
+
+/* Last error message; ctx may be NULL for errors raised before a context
+ * existed (thread-local). */
+const char* pd_last_error(const pd_context* ctx);
+
+/* Library / device identification ("gfx950 lib2dpdgpu x.y"). */
+const char* pd_version(void);
+
+/* Create a context.  X: 2n initial positions (xy interleaved), V: n volumes,
+ * families as CSR (row_ptr n+1, col nb) in the caller's particle ids.  Families
+ * must be symmetric (j in H_i <=> i in H_j) and hold at most 64 bonds. */
+int pd_create(const pd_params* p, int n, const double* X, const double* V,
+              const int* row_ptr, const int* col, pd_context** out);
+
+/* Same, from the reference GPU path's padded layout: nbr[n*cap] padded with
+ * INT_MAX, nbr_count[n] (include/mechanics.cuh:1033-1046). */
+int pd_create_padded(const pd_params* p, int n, const double* X, const double* V,
+                     const int* nbr, const int* nbr_count, int cap, pd_context** out);
+
+void pd_destroy(pd_context* ctx);
+
+/* Work on the caller's HIP stream (hipStream_t); NULL = the context's own. */
+int pd_set_stream(pd_context* ctx, void* hip_stream);
+int pd_synchronize(pd_context* ctx);
+int pd_set_option(pd_context* ctx, int option, long value);
+
+/* Per-particle Young's modulus (n values, caller order). */
+int pd_set_youngs_modulus(pd_context* ctx, const double* E);
+
+/* State in caller order, double[2n] each; NULL pointers are skipped.
+ * u = y - X (displacement), v velocity, a acceleration, f net force density. */
+int pd_set_state(pd_context* ctx, const double* u, const double* v, const double* a,
+                 const double* f);
+int pd_get_state(pd_context* ctx, double* u, double* v, double* a, double* f);
+/* Current positions y (double[2n]); u is formed as y - X. */
+int pd_set_positions(pd_context* ctx, const double* y);
+int pd_get_positions(pd_context* ctx, double* y);
+
+/* ---- phase calls mirroring the reference ------------------------------------ */
+/* applyVelocityBC: v = v0, a = 0 for ids (caller ids). */
+int pd_apply_velocity_bc(pd_context* ctx, int count, const int* ids, const double* v0);
+/* applyFixedBC: y = X (u = 0) for ids. */
+int pd_apply_fixed_bc(pd_context* ctx, int count, const int* ids);
+/* updatePositions: a = f/rho; u += v dt + 0.5 a dt^2, all particles. */
+int pd_update_positions(pd_context* ctx);
+/* computeVelocity: bond breaking (when s0 > 0), force density f_i =
+ * sum_j (T_ij - T_ji) V_j, then v += 0.5 (a + f/rho) dt.  write_acceleration
+ * != 0 also stores a = f/rho, as the reference GPU kernel does
+ * (include/mechanics.cuh:863); the CPU path does not. */
+int pd_compute_velocity(pd_context* ctx, int write_acceleration);
+
+/* ---- whole steps ------------------------------------------------------------ */
+/* Boundary program for pd_step: code[n] (caller order) with PD_BC_FREE,
+ * PD_BC_FIXED or a velocity group g in [0, n_groups); v0[2*n_groups]. */
+int pd_set_boundary(pd_context* ctx, const int* code, int n_groups, const double* v0);
+/* nsteps explicit steps: velocity BC -> updatePositions -> fixed BC ->
+ * bond breaking -> computeVelocity (src/main_seq.cpp:686-699 order). */
+int pd_step(pd_context* ctx, int nsteps);
+/* Times each launch-shape variant for `steps` steps on the context's own
+ * state (restored afterwards) and keeps the fastest. */
+int pd_autotune(pd_context* ctx, int steps);
+
+/* ---- bonds and diagnostics -------------------------------------------------- */
+/* Broken mask per directed bond in the caller's CSR order (nb bytes). */
+int pd_get_broken(pd_context* ctx, unsigned char* mask);
+int pd_set_broken(pd_context* ctx, const unsigned char* mask);
+/* Force state T_ij per directed bond (double[2*nb], caller CSR order) from the
+ * last force evaluation. */
+int pd_get_bond_forces(pd_context* ctx, double* T);
+int pd_get_stats(pd_context* ctx, pd_stats* out);
+void pd_reset_profile(pd_context* ctx);
+
+/* ---- setup helpers (host side; no counterpart in the hot path) -------------- */
+/* Families with the reference cutoff |xi| < 1.01*dx*horizon
+ * (src/SetupParticleSystem.cpp:113) from a cell list, neighbours in ascending
+ * id order like findNeighbor.  row_ptr[n+1]; col has room for cap entries.
+ * Returns nb (>= 0) or -1 when cap is too small / bad input. */
+long pd_build_families(int n, const double* X, double dx, double horizon,
+                       int* row_ptr, int* col, long cap);
+
+/* Pre-broken notch bonds: mask[b] = 1 when the segment X_i -> X_j of bond b
+ * properly crosses one of nseg segments seg[4s..4s+3] = (x0, y0, x1, y1).
+ * The reference makes notches by deleting particles
+ * (src/SetupParticleSystem.cpp:33); the configs pre-break crossing bonds.
+ * Symmetric in i, j.  Returns the number of marked directed bonds. */
+long pd_notch_mask(int n, const double* X, const int* row_ptr, const int* col,
+                   int nseg, const double* seg, unsigned char* mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,387 @@
+"""Pins the CPU oracle (oracle/pd_oracle.c) before it is trusted as the checker.
+
+* Reference fixtures: the Hassa mesh step of output/velocity_step_0_after_seq_ref.txt,
+  output/LocalParticlesPositions_rank_0step_0.txt, output/velocity_step_0_after_ref.txt
+  (MPI rank 0) and output/velocity_step_0_after_cuda.txt (with the reference GPU
+  kernel's y_l - X_i current-bond vector, include/mechanics.cuh:165).
+* The reference's own src/matrix.cpp, compiled into oracle/_ref (bit-for-bit).
+* Brute force: the literal O(N M^3) long-double restatement against the O(N M^2)
+  FP64 oracle on random displacements.
+* Closed forms and invariants: weight special cases, rigid translation, linearity,
+  linear momentum, uniform-strain patch test, Verlet kinematics, broken bonds ==
+  removed bonds, singular-K guard.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import hassa as H
+import oracle_lib as O
+
+
+# ---------------------------------------------------------------- helpers ----
+
+def grid(nx, ny, dx=1.0, jitter=0.0, seed=0):
+    rng = np.random.default_rng(seed)
+    iy, ix = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    X = np.empty(2 * nx * ny)
+    X[0::2] = ix.ravel() * dx + rng.uniform(-jitter, jitter, nx * ny) * dx
+    X[1::2] = iy.ravel() * dx + rng.uniform(-jitter, jitter, nx * ny) * dx
+    return X
+
+
+def families(X, dx, horizon):
+    Xl = X.astype(np.longdouble)
+    return O.ref_find_neighbor(Xl, np.longdouble(dx), np.longdouble(horizon))
+
+
+def small_params(**kw):
+    base = dict(horizon=3.015, dx=1.0, rho=1.0, dt=1e-3, E=1.0, nu=0.25, s0=0.0)
+    base.update(kw)
+    return O.or_params(**base)
+
+
+@pytest.fixture(scope="module")
+def hassa_case():
+    return H.case()
+
+
+@pytest.fixture(scope="module")
+def hassa_ref_step(hassa_case):
+    """One reference step on the Hassa mesh with the literal long-double restatement."""
+    X, V, row, col, sets = hassa_case
+    n = V.size
+    p = O.ref_params()
+    y = X.copy()
+    v = np.zeros(2 * n, np.longdouble)
+    a = np.zeros_like(v)
+    f = np.zeros_like(v)
+    O.ref_apply_velocity_bc(sets[3], H.V_TOP, v, a)
+    O.ref_update_positions(p, y, v, a, f)
+    O.ref_apply_fixed_bc(sets[2], X, y)
+    O.ref_compute_velocity(p, X, y, V, row, col, v, a, f, nthreads=8)
+    return y, v, f
+
+
+def hassa_oracle_step(hassa_case, bugcompat=False):
+    X, V, row, col, sets = hassa_case
+    n = V.size
+    po = O.or_params(horizon=H.HORIZON, dx=H.DX, rho=H.RHO, dt=H.DT, E=H.E, nu=H.NU)
+    Xd = X.astype(np.float64)
+    Vd = V.astype(np.float64)
+    bc = np.full(n, -1, np.int32)
+    bc[sets[3]] = 0
+    bc[sets[2]] = -2
+    rev = O.build_rev(row, col)
+    u = np.zeros(2 * n)
+    v = np.zeros(2 * n)
+    a = np.zeros(2 * n)
+    f = np.zeros(2 * n)
+    if not bugcompat:
+        O.step(po, Xd, Vd, row, col, rev, bc, np.array(H.V_TOP), u, v, a, f, None, 1)
+        return u, v, f
+    # the reference GPU kernel: positions updated on the host, then its kernel
+    u[2 * sets[3] + 1] = H.V_TOP[1] * H.DT
+    fb, _, _, _ = O.forces(po, Xd, Vd, row, col, u, rev=rev, bugcompat=True)
+    v[2 * sets[3] + 1] = H.V_TOP[1]
+    v = v + 0.5 * (fb / H.RHO) * H.DT
+    return u, v, fb
+
+
+# ------------------------------------------------------- reference fixtures --
+
+def test_hassa_mesh_matches_reference_counts(hassa_case):
+    X, V, row, col, sets = hassa_case
+    assert V.size == 4064
+    assert len(col) == 107996
+    m = np.diff(row)
+    assert m.min() == 10 and m.max() == 28
+    assert len(sets[2]) == 70 and len(sets[3]) == 70
+
+
+def test_ref_cpu_matches_seq_fixture(hassa_case, hassa_ref_step):
+    X, V, row, col, sets = hassa_case
+    _, v, _ = hassa_ref_step
+    vref = H.load_velocity_fixture("velocity_step_0_after_seq_ref.txt").reshape(-1)
+    mask = np.zeros(V.size, bool)
+    mask[sets[3]] = True
+    all_rel, nonbc, elem = H.parity_norms(v.astype(np.float64), vref, mask)
+    # the fixture holds 6 significant digits (include/util.h:29)
+    assert all_rel < 1e-8
+    assert nonbc < 5e-6
+    assert elem < 5e-6
+
+
+def test_positions_fixture(hassa_case, hassa_ref_step):
+    X, V, row, col, sets = hassa_case
+    y, _, _ = hassa_ref_step
+    fx = H.load_positions_fixture()
+    assert fx.shape[0] == V.size
+    np.testing.assert_array_equal(fx[:, 0], np.arange(V.size))
+    np.testing.assert_allclose(fx[:, 1], X[0::2].astype(float))
+    np.testing.assert_allclose(fx[:, 2], X[1::2].astype(float))
+    u = (y - X).astype(np.float64)
+    np.testing.assert_allclose(u[0::2], fx[:, 3], rtol=1e-5, atol=0)
+    np.testing.assert_allclose(u[1::2], fx[:, 4], rtol=1e-5, atol=1e-20)
+    assert np.count_nonzero(fx[:, 4]) == 70
+
+
+def test_fp64_oracle_matches_ref_cpu_and_fixture(hassa_case, hassa_ref_step):
+    X, V, row, col, sets = hassa_case
+    _, vref_ld, _ = hassa_ref_step
+    _, v, _ = hassa_oracle_step(hassa_case)
+    mask = np.zeros(V.size, bool)
+    mask[sets[3]] = True
+    # long-double positions carry ~1.7e-10 relative error in the top-row u
+    _, nonbc, elem = H.parity_norms(v, vref_ld.astype(np.float64), mask)
+    assert nonbc < 1e-9 and elem < 1e-9
+    vfix = H.load_velocity_fixture("velocity_step_0_after_seq_ref.txt").reshape(-1)
+    _, nonbc, elem = H.parity_norms(v, vfix, mask)
+    assert nonbc < 5e-6 and elem < 5e-6
+
+
+def test_mpi_rank0_rows_appear_in_oracle_step(hassa_case):
+    """The 246 rows written by MPI rank 0 (src/main.cpp:218) are a subset of the
+    sequential answer; each must be matched by some particle of the oracle's step."""
+    _, v, _ = hassa_oracle_step(hassa_case)
+    rows = H.load_velocity_fixture("velocity_step_0_after_ref.txt")
+    vv = v.reshape(-1, 2)
+    scale = np.abs(vv[:, 0]).max()
+    for r in rows:
+        d = np.abs(vv - r)
+        tol = 5e-6 * np.abs(r) + 1e-6 * scale
+        assert np.any(np.all(d <= tol + 1e-300, axis=1)), r
+
+
+def test_bug_compatible_mode_reproduces_cuda_fixture(hassa_case):
+    X, V, row, col, sets = hassa_case
+    _, vb, _ = hassa_oracle_step(hassa_case, bugcompat=True)
+    vc = H.load_velocity_fixture("velocity_step_0_after_cuda.txt").reshape(-1)
+    vs = H.load_velocity_fixture("velocity_step_0_after_seq_ref.txt").reshape(-1)
+    mask = np.zeros(V.size, bool)
+    mask[sets[3]] = True
+    _, nonbc, _ = H.parity_norms(vb, vc, mask)
+    assert nonbc < 1e-5
+    # and the reference's two outputs really differ (the divergence)
+    _, nonbc_seq_cuda, _ = H.parity_norms(vs, vc, mask)
+    assert nonbc_seq_cuda > 0.1
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_MATRIX_SO), reason="reference matrix.cpp not built")
+def test_matrix_ops_bit_exact_with_reference_matrix_cpp():
+    ref = C.CDLL(O.REF_MATRIX_SO)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        a = rng.normal(size=4).astype(np.longdouble) * np.longdouble(10.0) ** rng.integers(-12, 12)
+        b = rng.normal(size=4).astype(np.longdouble)
+        o1 = np.zeros(4, np.longdouble)
+        o2 = np.zeros(4, np.longdouble)
+        ref.refm_inverse2d(O._p(a), O._p(o1))
+        O.lib().ref_mat_inverse2d(O._p(a), O._p(o2))
+        assert np.array_equal(o1, o2)
+        ref.refm_mul(O._p(a), O._p(b), O._p(o1))
+        O.lib().ref_mat_mul(O._p(a), O._p(b), O._p(o2))
+        assert np.array_equal(o1, o2)
+
+
+# ------------------------------------------------------------ brute force ----
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_literal_restatement_equals_restructured_oracle(seed):
+    """O(N M^3) long-double loop nest == O(N M^2) displacement form."""
+    rng = np.random.default_rng(seed)
+    X = grid(9, 8, jitter=0.05, seed=seed)
+    n = X.size // 2
+    row, col = families(X, 1.0, 3.015)
+    V = rng.uniform(0.8, 1.2, n)
+    u = rng.normal(size=2 * n) * 1e-3
+    p = small_params(E=2.0, nu=0.3)
+    f, _, _, ns = O.forces(p, X, V, row, col, u)
+    assert ns == 0
+    rp = O.RefParams(p.n1, p.n2, p.horizon, p.dx, p.E, p.nu, 1.0, 1.0)
+    Xl = X.astype(np.longdouble)
+    yl = (X + u).astype(np.longdouble)
+    vl = np.zeros(2 * n, np.longdouble)
+    al = np.zeros(2 * n, np.longdouble)
+    fl = np.zeros(2 * n, np.longdouble)
+    O.ref_compute_velocity(rp, Xl, yl, V.astype(np.longdouble), row, col, vl, al, fl)
+    rel = np.linalg.norm(f - fl.astype(float)) / np.linalg.norm(f)
+    assert rel < 1e-10  # position-form cancellation in the literal code: |X|/|u| ~ 1e4
+
+
+# -------------------------------------------------- closed forms/invariants --
+
+def test_weight_special_cases():
+    p = O.or_params(horizon=3.0, dx=1.0)
+    assert O.weight(p, (1.0, 0.3), (1.0, 0.3)) == 1.0
+    assert O.weight(p, (1.0, 0.0), (-2.0, 0.0)) == 0.0
+    # perpendicular, equal lengths: (1/2)^10
+    assert abs(O.weight(p, (1.0, 0.0), (0.0, 1.0)) - 0.5 ** 10) < 1e-16
+    # parallel, lengths 1 and 2: exp(-10 * 1/3)
+    assert abs(O.weight(p, (1.0, 0.0), (2.0, 0.0)) - np.exp(-10.0 / 3.0)) < 1e-15
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        a, b = rng.normal(size=2), rng.normal(size=2)
+        assert O.weight(p, a, b) == pytest.approx(O.weight(p, b, a), rel=1e-15)
+
+
+@pytest.fixture(scope="module")
+def jittered():
+    rng = np.random.default_rng(3)
+    X = grid(14, 12, jitter=0.02, seed=3)
+    n = X.size // 2
+    row, col = families(X, 1.0, 3.015)
+    V = rng.uniform(0.9, 1.1, n)
+    return X, V, row, col
+
+
+def test_rigid_translation_gives_zero_force(jittered):
+    X, V, row, col = jittered
+    u = np.tile([3e-3, -7e-3], X.size // 2)
+    f, _, _, _ = O.forces(small_params(), X, V, row, col, u)
+    assert np.all(f == 0.0)
+
+
+def test_linearity(jittered):
+    X, V, row, col = jittered
+    rng = np.random.default_rng(4)
+    u1 = rng.normal(size=X.size) * 1e-3
+    u2 = rng.normal(size=X.size) * 1e-3
+    p = small_params()
+    f1, _, _, _ = O.forces(p, X, V, row, col, u1)
+    f2, _, _, _ = O.forces(p, X, V, row, col, u2)
+    f12, _, _, _ = O.forces(p, X, V, row, col, 2.0 * u1 - 3.0 * u2)
+    assert np.linalg.norm(f12 - (2 * f1 - 3 * f2)) < 1e-12 * np.linalg.norm(f12)
+
+
+def test_linear_momentum_balance(jittered):
+    X, V, row, col = jittered
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=X.size) * 1e-3
+    f, T, _, _ = O.forces(small_params(), X, V, row, col, u, want_T=True)
+    total = (f.reshape(-1, 2) * V[:, None]).sum(axis=0)
+    scale = np.abs(f.reshape(-1, 2) * V[:, None]).sum()
+    assert np.all(np.abs(total) < 1e-13 * scale)
+
+
+def test_uniform_strain_patch_interior_force_vanishes():
+    """Unjittered grid, u = eps0 X: every interior family is point-symmetric, so
+    the interior force density cancels; the boundary carries the traction."""
+    X = grid(21, 21)
+    n = X.size // 2
+    row, col = families(X, 1.0, 3.015)
+    V = np.ones(n)
+    eps = np.array([[1e-3, 2e-4], [2e-4, -5e-4]])
+    u = (X.reshape(-1, 2) @ eps.T).reshape(-1)
+    f, T, _, _ = O.forces(small_params(), X, V, row, col, u, want_T=True)
+    ff = f.reshape(-1, 2)
+    x, y = X[0::2], X[1::2]
+    inner = (x >= 7) & (x <= 13) & (y >= 7) & (y <= 13)
+    scale = np.abs(T).max()
+    assert np.abs(ff[inner]).max() < 1e-10 * scale
+    assert np.abs(ff[~inner]).max() > 1e-3 * scale
+
+
+def test_uniform_strain_bond_force_state_closed_form():
+    """For u = eps0 X every bond has F_ik - I = eps0 exactly (affine patch test:
+    D_ik = eps0 K_ik), so T_ij = (1/V_H) sum_k w(j,k) V_k sigma0 K_ik^-1 xi_ij
+    with the single plane-stress sigma0 = C eps0."""
+    X = grid(9, 9, jitter=0.03, seed=9)
+    n = X.size // 2
+    row, col = families(X, 1.0, 3.015)
+    V = np.ones(n)
+    eps = np.array([[1e-3, 3e-4], [1e-4, -5e-4]])
+    u = (X.reshape(-1, 2) @ eps.T).reshape(-1)
+    p = small_params(E=3.0, nu=0.2)
+    _, T, Ki, _ = O.forces(p, X, V, row, col, u, want_T=True, want_Kinv=True)
+    pf = p.E / (1 - p.nu ** 2)
+    e11, e22, e12 = eps[0, 0], eps[1, 1], 0.5 * (eps[0, 1] + eps[1, 0])
+    sig = np.array([[pf * (e11 + p.nu * e22), pf * (1 - p.nu) * e12],
+                    [pf * (1 - p.nu) * e12, pf * (p.nu * e11 + e22)]])
+    i = 40
+    b0, b1 = row[i], row[i + 1]
+    xi = X.reshape(-1, 2)[col[b0:b1]] - X.reshape(-1, 2)[i]
+    for jj in range(b1 - b0):
+        acc = np.zeros((2, 2))
+        for k in range(b1 - b0):
+            w = O.weight(p, xi[jj], xi[k])
+            kinv = Ki[3 * (b0 + k): 3 * (b0 + k) + 3]
+            Kinv = np.array([[kinv[0], kinv[1]], [kinv[1], kinv[2]]])
+            # K_ik^-1 itself checked against a direct assembly
+            K = np.zeros((2, 2))
+            for l in range(b1 - b0):
+                K += O.weight(p, xi[k], xi[l]) * np.outer(xi[l], xi[l])
+            np.testing.assert_allclose(Kinv @ K, np.eye(2), atol=1e-12)
+            acc += w * sig @ Kinv
+        t = acc @ xi[jj] / (b1 - b0)
+        np.testing.assert_allclose(T[2 * (b0 + jj): 2 * (b0 + jj) + 2], t, rtol=1e-11, atol=1e-17)
+
+
+def test_broken_bonds_equal_removed_bonds(jittered):
+    X, V, row, col = jittered
+    n = V.size
+    rng = np.random.default_rng(6)
+    u = rng.normal(size=X.size) * 1e-3
+    rev = O.build_rev(row, col)
+    broken = np.zeros(len(col), np.uint8)
+    pick = rng.choice(len(col), 40, replace=False)
+    broken[pick] = 1
+    broken[rev[pick]] = 1
+    p = small_params()
+    f1, _, _, _ = O.forces(p, X, V, row, col, u, broken=broken)
+    keep = broken == 0
+    counts = np.add.reduceat(keep.astype(np.int64), row[:-1]) if n else None
+    counts[np.diff(row) == 0] = 0
+    row2 = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    col2 = col[keep].astype(np.int32)
+    f2, _, _, _ = O.forces(p, X, V, row2, col2, u)
+    np.testing.assert_allclose(f1, f2, rtol=0, atol=1e-13 * np.abs(f2).max())
+
+
+def test_stretch_symmetric_and_breaking_rule():
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        xi = rng.normal(size=2)
+        du = rng.normal(size=2) * 0.03
+        assert O.bond_stretch(xi, du) == O.bond_stretch(-xi, -du)
+    X = grid(3, 1)
+    n = 3
+    row, col = families(X, 1.0, 3.015)
+    p = small_params(s0=0.02)
+    u = np.array([0.0, 0.0, 0.0, 0.0, 0.05, 0.0])  # bond 1-2 stretched 5 %, 0-2 2.5 %, 0-1 none
+    broken = np.zeros(len(col), np.uint8)
+    nnew = O.break_bonds(p, X, row, col, u, broken)
+    pairs = {(i, int(col[b])) for i in range(n) for b in range(row[i], row[i + 1]) if broken[b]}
+    assert pairs == {(1, 2), (2, 1), (0, 2), (2, 0)}
+    assert nnew == 4
+
+
+def test_singular_family_is_guarded():
+    X = grid(4, 1)  # collinear: every K_ik is rank one
+    row, col = families(X, 1.0, 3.015)
+    V = np.ones(4)
+    u = np.array([0, 0, 1e-3, 0, 0, 0, 0, 0], dtype=float)
+    f, _, _, ns = O.forces(small_params(), X, V, row, col, u)
+    assert ns == len(col)
+    assert np.all(np.isfinite(f)) and np.all(f == 0.0)
+
+
+def test_verlet_kinematics_closed_form():
+    """A particle with no bonds moves ballistically; a prescribed-velocity
+    particle is reset each step (src/main_seq.cpp:28-36, :386-393)."""
+    X = np.array([0.0, 0.0, 100.0, 0.0])
+    row = np.array([0, 0, 0], np.int32)
+    col = np.zeros(0, np.int32)
+    V = np.ones(2)
+    p = small_params(dt=1e-3)
+    u = np.zeros(4)
+    v = np.array([2.0, -1.0, 0.0, 0.0])
+    a = np.zeros(4)
+    f = np.zeros(4)
+    bc = np.array([-1, 0], np.int32)
+    O.step(p, X, V, row, col, np.zeros(0, np.int32), bc, np.array([0.5, 0.25]), u, v, a, f, None, 10)
+    np.testing.assert_allclose(u[:2], [2.0 * 10e-3, -1.0 * 10e-3], rtol=1e-14)
+    np.testing.assert_allclose(u[2:], [0.5 * 10e-3, 0.25 * 10e-3], rtol=1e-14)
+    np.testing.assert_array_equal(v, [2.0, -1.0, 0.5, 0.25])
