@@ -6,12 +6,15 @@ identical broken-bond sets.  Mixed precision: <= 1e-5 norm-wise velocity error
 against the FP64 oracle.  Norm-wise metric as in tests/hassa.parity_norms
 (per-element relative error is meaningless where velocities are round-off).
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
 import hassa as H
 import oracle_lib as O
-from conftest import say
+from conftest import ROOT, say
 
 pytestmark = pytest.mark.gpu
 
@@ -106,6 +109,36 @@ def test_hassa_mixed_precision_error(pdpkg, hassa_case, hassa_oracle):
     _, v, _, f, _ = hassa_oracle
     _, nonbc, _ = H.parity_norms(st["v"], v, bc_mask(hassa_case))
     assert nonbc < 1e-5
+
+
+def test_reference_signature_adapter_driver(hassa_case, tmp_path):
+    """A main.cu-style driver built against include/pd_mechanics_adapter.hpp
+    (compute_velocity_GPU_host / computeVelocity signatures) reproduces the
+    oracle given the same long-double position update."""
+    say("adapter driver (reference signatures)")
+    X, V, row, col, sets = hassa_case
+    exe = os.path.join(ROOT, "tests", "adapter", "build", "hassa_driver")
+    n = V.size
+    p = O.ref_params()
+    y = X.copy()
+    vl = np.zeros(2 * n, np.longdouble)
+    al = np.zeros_like(vl)
+    fl = np.zeros_like(vl)
+    O.ref_apply_velocity_bc(sets[3], H.V_TOP, vl, al)
+    O.ref_update_positions(p, y, vl, al, fl)
+    O.ref_apply_fixed_bc(sets[2], X, y)
+    u = (y - X).astype(np.float64)
+    po = O.or_params(horizon=H.HORIZON, dx=H.DX, rho=H.RHO, dt=H.DT, E=H.E, nu=H.NU)
+    f, _, _, _ = O.forces(po, X.astype(float), V.astype(float), row, col, u)
+    v0 = vl.astype(np.float64)
+    v = v0 + (0.5 * (al.astype(np.float64) + f / H.RHO)) * H.DT
+    for entry in ("gpu", "seq"):
+        out = tmp_path / f"v_{entry}.txt"
+        r = subprocess.run([exe, H.MESH, str(out), entry], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        vg = np.loadtxt(out).reshape(-1)
+        _, nonbc, elem = H.parity_norms(vg, v, bc_mask(hassa_case))
+        assert nonbc < 1e-12 and elem < 1e-12
 
 
 def test_padded_layout_equals_csr(pdpkg, hassa_case):
