@@ -32,6 +32,7 @@ struct pd_context {
     long nb = 0;
     int maxM = 0;
     int lpf = 32;
+    int bpl = 1;
     int n2_is_10 = 1;
     int dev = 0;
     std::vector<int> perm;   // internal -> caller
@@ -255,7 +256,7 @@ static int run_shape(pd_context* c)
     HIPCHK(c, launch_fill_u32(c->counters + 1, 0u, 1, c->stream));
     BondArgs a = bond_args(c);
     a.force_k = 1;
-    int rc = timed(c, 3, [&]() { return launch_shape(a, c->lpf, c->n2_is_10, c->wpb, c->stream); });
+    int rc = timed(c, 3, [&]() { return launch_shape(a, c->lpf, c->bpl, c->n2_is_10, c->wpb, c->stream); });
     if (rc) return rc;
     unsigned int h[2];
     HIPCHK(c, hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -534,11 +535,19 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
         c->wpb = (int)value;
         break;
     case PD_OPT_XCD_REMAP: c->xcd = value ? 8 : 1; break;
-    case PD_OPT_LANES_PER_FAM:
-        if (value == 0) c->lpf = c->maxM <= 32 ? 32 : 64;
-        else if (value == 64 || (value == 32 && c->maxM <= 32)) c->lpf = (int)value;
-        else return fail(c, PD_ERR_INVALID, "lanes per family must be 64, or 32 when families have <= 32 bonds");
+    case PD_OPT_TILE: {
+        int lpf = (int)(value / 100), bpl = (int)(value % 100);
+        if (value == 0) {
+            lpf = c->maxM <= 32 ? 32 : 64;
+            bpl = 1;
+        }
+        if (!tile_supported(c->p.precision == PD_MIXED, lpf, bpl) || lpf * bpl < c->maxM)
+            return fail(c, PD_ERR_INVALID, "tile %ld (lanes %d x bonds %d) unsupported for this precision "
+                        "or smaller than the largest family (%d)", value, lpf, bpl, c->maxM);
+        c->lpf = lpf;
+        c->bpl = bpl;
         break;
+    }
     default: return fail(c, PD_ERR_INVALID, "unknown option %d", opt);
     }
     c->graph_valid = false;
@@ -665,7 +674,7 @@ static int force_and_velocity(pd_context* c, int write_a)
     BondArgs a = bond_args(c);
     GatherArgs g = gather_args(c, write_a);
     int mixed = c->p.precision == PD_MIXED;
-    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->n2_is_10, c->wpb, c->stream); }))
+    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->n2_is_10, c->wpb, c->stream); }))
         return rc;
     return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream); });
 }
@@ -807,6 +816,7 @@ extern "C" int pd_get_stats(pd_context* c, pd_stats* s)
     s->nb = c->nb;
     s->max_family = c->maxM;
     s->lanes_per_family = c->lpf;
+    s->bonds_per_lane = c->bpl;
     s->steps = c->steps;
     s->n_singular = c->n_singular;
     std::vector<unsigned char> t(c->nb);
@@ -879,19 +889,22 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     const int saved_profile = c->profile;
     const long saved_steps = c->steps;
     c->profile = 0;
-    struct Var { int wpb, xcd, lpf; };
+    struct Var { int wpb, xcd, lpf, bpl; };
     std::vector<Var> vars;
-    for (int lpf : {32, 64}) {
-        if (lpf == 32 && c->maxM > 32) continue;
+    const int mixed = c->p.precision == PD_MIXED;
+    const int tiles[4][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}};
+    for (const auto& tl : tiles) {
+        if (!tile_supported(mixed, tl[0], tl[1]) || tl[0] * tl[1] < c->maxM) continue;
         for (int w : {1, 2, 4, 8})
-            for (int x : {1, 8}) vars.push_back({w, x, lpf});
+            for (int x : {1, 8}) vars.push_back({w, x, tl[0], tl[1]});
     }
     double best = 1e300;
-    Var bestv = {c->wpb, c->xcd, c->lpf};
+    Var bestv = {c->wpb, c->xcd, c->lpf, c->bpl};
     for (const Var& vv : vars) {
         c->wpb = vv.wpb;
         c->xcd = vv.xcd;
         c->lpf = vv.lpf;
+        c->bpl = vv.bpl;
         c->graph_valid = false;
         if ((rc = pd_step(c, 1))) break;  // warm-up
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -912,6 +925,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     c->wpb = bestv.wpb;
     c->xcd = bestv.xcd;
     c->lpf = bestv.lpf;
+    c->bpl = bestv.bpl;
     c->graph_valid = false;
     c->profile = saved_profile;
     c->steps = saved_steps;

@@ -57,9 +57,12 @@ struct PosArgs {
     double rho, dt;
 };
 
-hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int n2_is_10, int wpb,
+// Tiles: (lanes per family, bonds per lane) in {(32,1), (64,1), (16,2)} and,
+// for mixed precision, (32,2); a family may hold up to lpf * bpl bonds.
+bool tile_supported(int mixed, int lpf, int bpl);
+hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int n2_is_10, int wpb,
                        hipStream_t s);
-hipError_t launch_shape(const BondArgs& a, int lpf, int n2_is_10, int wpb, hipStream_t s);
+hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int wpb, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, int mixed, hipStream_t s);
 hipError_t launch_positions(const PosArgs& a, hipStream_t s);
 hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double2* v,

@@ -96,215 +96,305 @@ __device__ __forceinline__ double family_sum(double x)
     return x;
 }
 
-// FP64 assembly of K_ik over the alive bonds of each family in the wave, and
-// its inverse under the scale-relative singularity rule.  rec: this wave's LDS
-// records (64 x 8 doubles).  Every lane of the wave must call it.
-template <int LPF, int N2_IS_10>
-__device__ __forceinline__ void assemble_kinv(double* rec, int lane, int half, bool act, bool alive,
-                                              double ex, double ey, double pe, double qe,
-                                              double Vxx, double Vxy, double Vyy, double n2,
-                                              double ktol, double& k00, double& k01, double& k11,
-                                              bool& singular)
-{
-    double* my = rec + lane * 8;
-    my[0] = ex;
-    my[1] = ey;
-    my[2] = pe;
-    my[3] = qe;
-    my[4] = alive ? Vxx : 0.0;
-    my[5] = alive ? Vxy : 0.0;
-    my[6] = alive ? Vyy : 0.0;
-    wave_sync();
-    const double* base = rec + half * (32 * 8);
-    double K00 = 0.0, K01 = 0.0, K11 = 0.0;
-#pragma unroll 2
-    for (int l = 0; l < LPF; ++l) {
-        const double* r = base + l * 8;
-        double w = weight<double, N2_IS_10>(ex, ey, pe, qe, r[0], r[1], r[2], r[3], n2);
-        K00 += w * r[4];
-        K01 += w * r[5];
-        K11 += w * r[6];
-    }
-    wave_sync();
-    double det = K00 * K11 - K01 * K01;
-    double tr = K00 + K11;
-    singular = false;
-    if (alive && det > ktol * tr * tr) {
-        k00 = K11 / det;
-        k01 = -K01 / det;
-        k11 = K00 / det;
-    } else {
-        k00 = k01 = k11 = 0.0;
-        singular = alive;
-    }
-    (void)act;
-}
-
 template <typename R> struct Vec2;
 template <> struct Vec2<double> { typedef double2 T; };
 template <> struct Vec2<float> { typedef float2 T; };
 
-// Per-lane bond setup shared by the shape and bond kernels.
-struct BondLane {
-    int fam, k, b, j, M;
-    bool famok, act;
-    double x0, x1, d0, d1, len, Vj, Vi;
+// Launch tile: LPF lanes per family, BPL bonds per lane (lane t owns bonds
+// t, t + LPF, ...), so a family has FS = LPF * BPL slots and a wave holds
+// FPW = 64 / LPF families.  More bonds per lane cut the LDS broadcast traffic
+// per weight evaluation (each record read feeds BPL rows) at the cost of
+// registers for BPL weight rows.
+template <int LPF, int BPL> struct Tile {
+    static constexpr int FS = LPF * BPL;
+    static constexpr int FPW = 64 / LPF;
+    static constexpr int REC = 64 * BPL;  // LDS records (8 values each) per wave
+};
+
+// The family a lane works on.
+struct LaneFam {
+    int fam, t, slot, beg, M;
+    bool famok;
+    double2 Xi, ui;
+    double Vi;
 };
 
 template <int LPF>
-__device__ __forceinline__ BondLane load_lane(const BondArgs& a, int lane, int half, int wave_global)
+__device__ __forceinline__ LaneFam load_family(const BondArgs& a, int lane, int wave_global)
 {
-    constexpr int FPW = 64 / LPF;
-    BondLane L;
-    L.k = (LPF == 32) ? (lane & 31) : lane;
-    L.fam = wave_global * FPW + half;
-    L.famok = L.fam < a.n;
-    const int fi = L.famok ? L.fam : 0;
-    int beg = 0, M = 0;
-    if (L.famok) {
-        beg = a.row[fi];
-        M = a.row[fi + 1] - beg;
+    LaneFam F;
+    F.slot = lane / LPF;
+    F.t = lane % LPF;
+    F.fam = wave_global * (64 / LPF) + F.slot;
+    F.famok = F.fam < a.n;
+    const int fi = F.famok ? F.fam : 0;
+    F.beg = 0;
+    F.M = 0;
+    if (F.famok) {
+        F.beg = a.row[fi];
+        F.M = a.row[fi + 1] - F.beg;
     }
-    L.M = M;
-    L.act = L.famok && L.k < M;
-    L.b = beg + L.k;
-    L.j = L.act ? a.col[L.b] : fi;
-    const double2 Xi = a.X[fi], Xj = a.X[L.j];
-    const double2 ui = a.u[fi], uj = a.u[L.j];
-    L.x0 = Xj.x - Xi.x;
-    L.x1 = Xj.y - Xi.y;
-    L.d0 = uj.x - ui.x;
-    L.d1 = uj.y - ui.y;
-    L.len = sqrt(L.x0 * L.x0 + L.x1 * L.x1);
-    L.Vj = a.V[L.j];
-    L.Vi = a.V[fi];
-    return L;
+    F.Xi = a.X[fi];
+    F.ui = a.u[fi];
+    F.Vi = a.V[fi];
+    return F;
+}
+
+// One bond slot of a lane.
+struct BondSlot {
+    int b, k;
+    bool act;
+    double x0, x1, d0, d1, len, Vj;
+};
+
+__device__ __forceinline__ BondSlot load_slot(const BondArgs& a, const LaneFam& F, int k)
+{
+    BondSlot S;
+    S.k = k;
+    S.act = F.famok && k < F.M;
+    S.b = F.beg + k;
+    const int j = S.act ? a.col[S.b] : (F.famok ? F.fam : 0);
+    const double2 Xj = a.X[j], uj = a.u[j];
+    S.x0 = Xj.x - F.Xi.x;
+    S.x1 = Xj.y - F.Xi.y;
+    S.d0 = uj.x - F.ui.x;
+    S.d1 = uj.y - F.ui.y;
+    S.len = sqrt(S.x0 * S.x0 + S.x1 * S.x1);
+    S.Vj = a.V[j];
+    return S;
+}
+
+// Bond geometry for the weight: unit vector and radial factors (FP64).
+struct Geo {
+    double ex, ey, pe, qe;
+};
+
+__device__ __forceinline__ Geo geometry(const BondArgs& a, const BondSlot& S)
+{
+    Geo g;
+    const double il = S.act ? 1.0 / S.len : 0.0;
+    g.ex = S.x0 * il;
+    g.ey = S.x1 * il;
+    g.pe = S.act ? exp(a.A * S.len) : 0.0;
+    g.qe = S.act ? exp(-a.A * S.len) : 0.0;
+    return g;
+}
+
+// FP64 assembly of K_ik over the alive bonds of each family in the wave, and
+// its inverse under the scale-relative singularity rule.  rec: this wave's LDS
+// records (Tile::REC x 8 doubles).  Every lane of the wave must call it.
+template <int LPF, int BPL, int N2_IS_10>
+__device__ __forceinline__ void assemble_kinv(double* rec, const LaneFam& F, const BondSlot* S,
+                                              const bool* alive, const Geo* g, double n2,
+                                              double ktol, double (*kinv)[3], bool* singular)
+{
+    constexpr int FS = Tile<LPF, BPL>::FS;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        double* my = rec + (F.slot * FS + S[s].k) * 8;
+        const double V = alive[s] ? S[s].Vj : 0.0;
+        my[0] = g[s].ex;
+        my[1] = g[s].ey;
+        my[2] = g[s].pe;
+        my[3] = g[s].qe;
+        my[4] = V * S[s].x0 * S[s].x0;
+        my[5] = V * S[s].x0 * S[s].x1;
+        my[6] = V * S[s].x1 * S[s].x1;
+    }
+    wave_sync();
+    const double* base = rec + F.slot * FS * 8;
+    double K[BPL][3];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) K[s][0] = K[s][1] = K[s][2] = 0.0;
+#pragma unroll 2
+    for (int l = 0; l < FS; ++l) {
+        const double* r = base + l * 8;
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) {
+            double w = weight<double, N2_IS_10>(g[s].ex, g[s].ey, g[s].pe, g[s].qe, r[0], r[1], r[2],
+                                                r[3], n2);
+            K[s][0] += w * r[4];
+            K[s][1] += w * r[5];
+            K[s][2] += w * r[6];
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        const double det = K[s][0] * K[s][2] - K[s][1] * K[s][1];
+        const double tr = K[s][0] + K[s][2];
+        singular[s] = false;
+        if (alive[s] && det > ktol * tr * tr) {
+            kinv[s][0] = K[s][2] / det;
+            kinv[s][1] = -K[s][1] / det;
+            kinv[s][2] = K[s][0] / det;
+        } else {
+            kinv[s][0] = kinv[s][1] = kinv[s][2] = 0.0;
+            singular[s] = alive[s];
+        }
+    }
+}
+
+template <int LPF>
+__device__ __forceinline__ int wave_max_family(int m)
+{
+#pragma unroll
+    for (int off = LPF; off < 64; off <<= 1) m = max(m, __shfl_xor(m, off, 64));
+    return __builtin_amdgcn_readfirstlane(m);
 }
 
 // K_ik^-1 for every bond of every family (context creation, pd_set_broken).
-template <int LPF, int N2_IS_10>
+template <int LPF, int BPL, int N2_IS_10>
 __global__ void __launch_bounds__(512) k_shape(BondArgs a)
 {
     extern __shared__ double smem[];
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    const int half = (LPF == 32) ? (lane >> 5) : 0;
     const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
-    BondLane L = load_lane<LPF>(a, lane, half, blk * wpb + wib);
-    const bool alive = L.act && a.broken[L.b] == 0;
-    const double il = L.act ? 1.0 / L.len : 0.0;
-    const double ex = L.x0 * il, ey = L.x1 * il;
-    const double pe = L.act ? exp(a.A * L.len) : 0.0;
-    const double qe = L.act ? exp(-a.A * L.len) : 0.0;
-    double k00, k01, k11;
-    bool sing;
-    assemble_kinv<LPF, N2_IS_10>(smem + wib * 64 * 8, lane, half, L.act, alive, ex, ey, pe, qe,
-                                 L.Vj * L.x0 * L.x0, L.Vj * L.x0 * L.x1, L.Vj * L.x1 * L.x1,
-                                 a.n2, a.ktol, k00, k01, k11, sing);
-    if (L.act) {
-        a.kinv[3 * (size_t)L.b] = k00;
-        a.kinv[3 * (size_t)L.b + 1] = k01;
-        a.kinv[3 * (size_t)L.b + 2] = k11;
+    const LaneFam F = load_family<LPF>(a, lane, blk * wpb + wib);
+    BondSlot S[BPL];
+    bool alive[BPL];
+    Geo g[BPL];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        S[s] = load_slot(a, F, F.t + s * LPF);
+        alive[s] = S[s].act && a.broken[S[s].b] == 0;
+        g[s] = geometry(a, S[s]);
     }
-    const uint64_t sm = __ballot(sing);
-    if (lane == 0 && sm) atomicAdd(a.counters + 1, (unsigned int)__popcll(sm));
+    double kinv[BPL][3];
+    bool sing[BPL];
+    assemble_kinv<LPF, BPL, N2_IS_10>(smem + wib * Tile<LPF, BPL>::REC * 8, F, S, alive, g, a.n2,
+                                      a.ktol, kinv, sing);
+    int nsing = 0;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        if (S[s].act) {
+            a.kinv[3 * (size_t)S[s].b] = kinv[s][0];
+            a.kinv[3 * (size_t)S[s].b + 1] = kinv[s][1];
+            a.kinv[3 * (size_t)S[s].b + 2] = kinv[s][2];
+        }
+        nsing += __popcll(__ballot(sing[s]));
+    }
+    if (lane == 0 && nsing) atomicAdd(a.counters + 1, (unsigned int)nsing);
 }
 
 // One explicit force evaluation: bond breaking, K re-assembly for families that
 // lost a bond, then the two family contractions and the force-state outputs.
-template <typename R, int LPF, int N2_IS_10>
+template <typename R, int LPF, int BPL, int N2_IS_10>
 __global__ void __launch_bounds__(512) k_bond(BondArgs a)
 {
     typedef typename Vec2<R>::T R2;
+    constexpr int FS = Tile<LPF, BPL>::FS;
     extern __shared__ double smem[];
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    const int half = (LPF == 32) ? (lane >> 5) : 0;
     const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
-    BondLane L = load_lane<LPF>(a, lane, half, blk * wpb + wib);
-    double* rec64 = smem + wib * 64 * 8;
+    const LaneFam F = load_family<LPF>(a, lane, blk * wpb + wib);
+    double* rec64 = smem + wib * Tile<LPF, BPL>::REC * 8;
 
-    // ---- bond breaking (FP64, irreversible, both directions agree) ----
-    bool alive = L.act && a.broken[L.b] == 0;
-    bool newly = false;
-    if (a.s0 > 0.0 && alive) {
-        double s = bond_stretch(L.x0, L.x1, L.d0, L.d1);
-        if (s > a.s0) {
-            alive = false;
-            newly = true;
-            a.broken[L.b] = 1;
+    BondSlot S[BPL];
+    bool alive[BPL];
+    int nnew = 0;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        S[s] = load_slot(a, F, F.t + s * LPF);
+        // ---- bond breaking (FP64, irreversible, both directions agree) ----
+        alive[s] = S[s].act && a.broken[S[s].b] == 0;
+        bool newly = false;
+        if (a.s0 > 0.0 && alive[s]) {
+            const double st = bond_stretch(S[s].x0, S[s].x1, S[s].d0, S[s].d1);
+            if (st > a.s0) {
+                alive[s] = false;
+                newly = true;
+                a.broken[S[s].b] = 1;
+            }
         }
+        nnew += __popcll(__ballot(newly));
     }
-    const uint64_t nm = __ballot(newly);
-    if (lane == 0 && nm) atomicAdd(a.counters, (unsigned int)__popcll(nm));
-
-    // ---- geometry (FP64) ----
-    const double il = L.act ? 1.0 / L.len : 0.0;
-    const double ex = L.x0 * il, ey = L.x1 * il;
-    const double pe = L.act ? exp(a.A * L.len) : 0.0;
-    const double qe = L.act ? exp(-a.A * L.len) : 0.0;
+    const int nm = nnew;
+    if (lane == 0 && nnew) atomicAdd(a.counters, (unsigned int)nnew);
+    Geo g[BPL];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) g[s] = geometry(a, S[s]);
 
     // ---- K_ik^-1: stored, or re-assembled in FP64 when the family changed ----
-    double k00 = 0.0, k01 = 0.0, k11 = 0.0;
+    double kinv[BPL][3];
     if (nm != 0 || a.force_k) {
-        bool sing;
-        assemble_kinv<LPF, N2_IS_10>(rec64, lane, half, L.act, alive, ex, ey, pe, qe,
-                                     L.Vj * L.x0 * L.x0, L.Vj * L.x0 * L.x1, L.Vj * L.x1 * L.x1,
-                                     a.n2, a.ktol, k00, k01, k11, sing);
-        if (L.act) {
-            a.kinv[3 * (size_t)L.b] = k00;
-            a.kinv[3 * (size_t)L.b + 1] = k01;
-            a.kinv[3 * (size_t)L.b + 2] = k11;
+        bool sing[BPL];
+        assemble_kinv<LPF, BPL, N2_IS_10>(rec64, F, S, alive, g, a.n2, a.ktol, kinv, sing);
+        int nsing = 0;
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) {
+            if (S[s].act) {
+                a.kinv[3 * (size_t)S[s].b] = kinv[s][0];
+                a.kinv[3 * (size_t)S[s].b + 1] = kinv[s][1];
+                a.kinv[3 * (size_t)S[s].b + 2] = kinv[s][2];
+            }
+            nsing += __popcll(__ballot(sing[s]));
         }
-        const uint64_t sm = __ballot(sing);
-        if (lane == 0 && sm) atomicAdd(a.counters + 1, (unsigned int)__popcll(sm));
-    } else if (alive) {
-        k00 = a.kinv[3 * (size_t)L.b];
-        k01 = a.kinv[3 * (size_t)L.b + 1];
-        k11 = a.kinv[3 * (size_t)L.b + 2];
+        if (lane == 0 && nsing) atomicAdd(a.counters + 1, (unsigned int)nsing);
+    } else {
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) {
+            kinv[s][0] = kinv[s][1] = kinv[s][2] = 0.0;
+            if (alive[s]) {
+                kinv[s][0] = a.kinv[3 * (size_t)S[s].b];
+                kinv[s][1] = a.kinv[3 * (size_t)S[s].b + 1];
+                kinv[s][2] = a.kinv[3 * (size_t)S[s].b + 2];
+            }
+        }
     }
 
     // ---- records for the first contraction ----
     R* rec = reinterpret_cast<R*>(rec64);
-    const R exr = R(ex), eyr = R(ey), per = R(pe), qer = R(qe);
-    const R x0r = R(L.x0), x1r = R(L.x1);
-    const R Vd0 = alive ? R(L.Vj * L.d0) : R(0), Vd1 = alive ? R(L.Vj * L.d1) : R(0);
-    {
-        R* my = rec + lane * 8;
-        my[0] = exr;
-        my[1] = eyr;
-        my[2] = per;
-        my[3] = qer;
+    R gx[BPL], gy[BPL], gp[BPL], gq[BPL];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        gx[s] = R(g[s].ex);
+        gy[s] = R(g[s].ey);
+        gp[s] = R(g[s].pe);
+        gq[s] = R(g[s].qe);
+        const R x0r = R(S[s].x0), x1r = R(S[s].x1);
+        const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) : R(0);
+        const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) : R(0);
+        R* my = rec + (F.slot * FS + S[s].k) * 8;
+        my[0] = gx[s];
+        my[1] = gy[s];
+        my[2] = gp[s];
+        my[3] = gq[s];
         my[4] = Vd0 * x0r;
         my[5] = Vd0 * x1r;
         my[6] = Vd1 * x0r;
         my[7] = Vd1 * x1r;
     }
-    // loop bound: largest family in the wave
-    int mfam = L.M;
-    if (LPF == 32) mfam = max(mfam, __shfl_xor(mfam, 32, 64));
-    const int mmax = __builtin_amdgcn_readfirstlane(mfam);
+    const int mmax = wave_max_family<LPF>(F.M);
     wave_sync();
 
     // ---- contraction 1: D_ik = sum_l w(k,l) V_l du_l (x) xi_l ----
-    const R* base = rec + half * (32 * 8);
+    const R* base = rec + F.slot * FS * 8;
     const R n2r = R(a.n2);
-    R w[LPF];
-    R D00 = 0, D01 = 0, D10 = 0, D11 = 0;
+    R w[BPL][FS];
+    R D[BPL][4];
 #pragma unroll
-    for (int l = 0; l < LPF; ++l) {
+    for (int s = 0; s < BPL; ++s) D[s][0] = D[s][1] = D[s][2] = D[s][3] = R(0);
+#pragma unroll
+    for (int l = 0; l < FS; ++l) {
         if (l < mmax) {
             const R* r = base + l * 8;
-            const R om = weight<R, N2_IS_10>(exr, eyr, per, qer, r[0], r[1], r[2], r[3], n2r);
-            w[l] = om;
-            D00 += om * r[4];
-            D01 += om * r[5];
-            D10 += om * r[6];
-            D11 += om * r[7];
+#pragma unroll
+            for (int s = 0; s < BPL; ++s) {
+                const R om = weight<R, N2_IS_10>(gx[s], gy[s], gp[s], gq[s], r[0], r[1], r[2], r[3], n2r);
+                w[s][l] = om;
+                D[s][0] += om * r[4];
+                D[s][1] += om * r[5];
+                D[s][2] += om * r[6];
+                D[s][3] += om * r[7];
+            }
         } else {
-            w[l] = R(0);
+#pragma unroll
+            for (int s = 0; s < BPL; ++s) w[s][l] = R(0);
         }
         // keep the scheduler from hoisting every record load of the unrolled
         // loop (which would spill)
@@ -312,63 +402,88 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     }
 
     // ---- F - I = D K^-1, plane-stress sigma, Q_k = V_k sigma K^-1 ----
-    const double Ei = a.E ? a.E[L.famok ? L.fam : 0] : a.E0;
+    const double Ei = a.E ? a.E[F.famok ? F.fam : 0] : a.E0;
     const double pf = Ei / (1.0 - a.nu * a.nu);
     const R c11 = R(pf), c12 = R(pf * a.nu), c33 = R(pf * (1.0 - a.nu));
-    const R K00 = R(k00), K01 = R(k01), K11 = R(k11);
-    const R G00 = D00 * K00 + D01 * K01;
-    const R G01 = D00 * K01 + D01 * K11;
-    const R G10 = D10 * K00 + D11 * K01;
-    const R G11 = D10 * K01 + D11 * K11;
-    const R e11 = G00, e22 = G11, e12 = R(0.5) * (G01 + G10);
-    const R s11 = c11 * e11 + c12 * e22;
-    const R s22 = c12 * e11 + c11 * e22;
-    const R s12 = c33 * e12;
-    const R Vk = alive ? R(L.Vj) : R(0);
-    const R Q00 = (s11 * K00 + s12 * K01) * Vk;
-    const R Q01 = (s11 * K01 + s12 * K11) * Vk;
-    const R Q10 = (s12 * K00 + s22 * K01) * Vk;
-    const R Q11 = (s12 * K01 + s22 * K11) * Vk;
+    R Q[BPL][4];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        const R K00 = R(kinv[s][0]), K01 = R(kinv[s][1]), K11 = R(kinv[s][2]);
+        const R G00 = D[s][0] * K00 + D[s][1] * K01;
+        const R G01 = D[s][0] * K01 + D[s][1] * K11;
+        const R G10 = D[s][2] * K00 + D[s][3] * K01;
+        const R G11 = D[s][2] * K01 + D[s][3] * K11;
+        const R e11 = G00, e22 = G11, e12 = R(0.5) * (G01 + G10);
+        const R s11 = c11 * e11 + c12 * e22;
+        const R s22 = c12 * e11 + c11 * e22;
+        const R s12 = c33 * e12;
+        const R Vk = alive[s] ? R(S[s].Vj) : R(0);
+        Q[s][0] = (s11 * K00 + s12 * K01) * Vk;
+        Q[s][1] = (s11 * K01 + s12 * K11) * Vk;
+        Q[s][2] = (s12 * K00 + s22 * K01) * Vk;
+        Q[s][3] = (s12 * K01 + s22 * K11) * Vk;
+    }
     wave_sync();
-    {
-        R* my = rec + lane * 8;
-        my[4] = Q00;
-        my[5] = Q01;
-        my[6] = Q10;
-        my[7] = Q11;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        R* my = rec + (F.slot * FS + S[s].k) * 8;
+        my[4] = Q[s][0];
+        my[5] = Q[s][1];
+        my[6] = Q[s][2];
+        my[7] = Q[s][3];
     }
     wave_sync();
 
     // ---- contraction 2: S_ij = sum_k w(j,k) V_k sigma_ik K_ik^-1 ----
-    R S00 = 0, S01 = 0, S10 = 0, S11 = 0;
+    R Sm[BPL][4];
 #pragma unroll
-    for (int l = 0; l < LPF; ++l) {
+    for (int s = 0; s < BPL; ++s) Sm[s][0] = Sm[s][1] = Sm[s][2] = Sm[s][3] = R(0);
+#pragma unroll
+    for (int l = 0; l < FS; ++l) {
         if (l < mmax) {
             const R* r = base + l * 8;
-            S00 += w[l] * r[4];
-            S01 += w[l] * r[5];
-            S10 += w[l] * r[6];
-            S11 += w[l] * r[7];
+#pragma unroll
+            for (int s = 0; s < BPL; ++s) {
+                Sm[s][0] += w[s][l] * r[4];
+                Sm[s][1] += w[s][l] * r[5];
+                Sm[s][2] += w[s][l] * r[6];
+                Sm[s][3] += w[s][l] * r[7];
+            }
         }
         if ((l & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
-    const double VH = family_sum<LPF>(alive ? L.Vj : 0.0);
-    R T0 = 0, T1 = 0;
-    if (alive) {
-        const R ivh = R(1.0 / VH);
-        T0 = (S00 * x0r + S01 * x1r) * ivh;
-        T1 = (S10 * x0r + S11 * x1r) * ivh;
+    double vsum = 0.0;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) vsum += alive[s] ? S[s].Vj : 0.0;
+    const double VH = family_sum<LPF>(vsum);
+    R T0[BPL], T1[BPL];
+    double fo0 = 0.0, fo1 = 0.0;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        T0[s] = R(0);
+        T1[s] = R(0);
+        if (alive[s]) {
+            const R ivh = R(1.0 / VH);
+            const R x0r = R(S[s].x0), x1r = R(S[s].x1);
+            T0[s] = (Sm[s][0] * x0r + Sm[s][1] * x1r) * ivh;
+            T1[s] = (Sm[s][2] * x0r + Sm[s][3] * x1r) * ivh;
+        }
+        fo0 += double(T0[s]) * S[s].Vj;
+        fo1 += double(T1[s]) * S[s].Vj;
     }
 
     // ---- outputs: own-bond sum and the reverse-slot terms ----
-    const double fo0 = family_sum<LPF>(double(T0) * L.Vj);
-    const double fo1 = family_sum<LPF>(double(T1) * L.Vj);
-    if (L.famok && L.k == 0) a.fown[L.fam] = make_double2(fo0, fo1);
-    if (L.act) {
-        R2 t;
-        t.x = T0 * R(L.Vi);
-        t.y = T1 * R(L.Vi);
-        reinterpret_cast<R2*>(a.W)[a.rev[L.b]] = t;
+    fo0 = family_sum<LPF>(fo0);
+    fo1 = family_sum<LPF>(fo1);
+    if (F.famok && F.t == 0) a.fown[F.fam] = make_double2(fo0, fo1);
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        if (S[s].act) {
+            R2 t;
+            t.x = T0[s] * R(F.Vi);
+            t.y = T1[s] * R(F.Vi);
+            reinterpret_cast<R2*>(a.W)[a.rev[S[s].b]] = t;
+        }
     }
 }
 
@@ -455,42 +570,62 @@ __global__ void k_fill_u32(unsigned int* p, unsigned int v, int count)
 // ---------------------------------------------------------------------------
 static inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
-template <typename R, int LPF, int N2>
+template <typename R, int LPF, int BPL, int N2>
 static hipError_t launch_bond_t(const BondArgs& a, int wpb, hipStream_t s)
 {
-    const int fpw = 64 / LPF;
-    const int grid = ceil_div(a.n, (long)wpb * fpw);
+    const int grid = ceil_div(a.n, (long)wpb * Tile<LPF, BPL>::FPW);
     if (grid == 0) return hipSuccess;
-    const size_t lds = (size_t)wpb * 64 * 8 * sizeof(double);
-    hipLaunchKernelGGL((k_bond<R, LPF, N2>), dim3(grid), dim3(64 * wpb), lds, s, a);
+    const size_t lds = (size_t)wpb * Tile<LPF, BPL>::REC * 8 * sizeof(double);
+    hipLaunchKernelGGL((k_bond<R, LPF, BPL, N2>), dim3(grid), dim3(64 * wpb), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int n2_is_10, int wpb, hipStream_t s)
+template <typename R, int LPF, int BPL>
+static hipError_t launch_bond_n2(const BondArgs& a, int n2_is_10, int wpb, hipStream_t s)
+{
+    return n2_is_10 ? launch_bond_t<R, LPF, BPL, 1>(a, wpb, s) : launch_bond_t<R, LPF, BPL, 0>(a, wpb, s);
+}
+
+bool tile_supported(int mixed, int lpf, int bpl)
+{
+    if (bpl == 1) return lpf == 32 || lpf == 64;
+    if (bpl == 2) return lpf == 16 || (lpf == 32 && mixed);
+    return false;
+}
+
+hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int n2_is_10, int wpb,
+                       hipStream_t s)
 {
     if (mixed) {
-        if (lpf == 32) return n2_is_10 ? launch_bond_t<float, 32, 1>(a, wpb, s) : launch_bond_t<float, 32, 0>(a, wpb, s);
-        return n2_is_10 ? launch_bond_t<float, 64, 1>(a, wpb, s) : launch_bond_t<float, 64, 0>(a, wpb, s);
+        if (bpl == 1 && lpf == 32) return launch_bond_n2<float, 32, 1>(a, n2_is_10, wpb, s);
+        if (bpl == 1 && lpf == 64) return launch_bond_n2<float, 64, 1>(a, n2_is_10, wpb, s);
+        if (bpl == 2 && lpf == 16) return launch_bond_n2<float, 16, 2>(a, n2_is_10, wpb, s);
+        if (bpl == 2 && lpf == 32) return launch_bond_n2<float, 32, 2>(a, n2_is_10, wpb, s);
+    } else {
+        if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1>(a, n2_is_10, wpb, s);
+        if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1>(a, n2_is_10, wpb, s);
+        if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2>(a, n2_is_10, wpb, s);
     }
-    if (lpf == 32) return n2_is_10 ? launch_bond_t<double, 32, 1>(a, wpb, s) : launch_bond_t<double, 32, 0>(a, wpb, s);
-    return n2_is_10 ? launch_bond_t<double, 64, 1>(a, wpb, s) : launch_bond_t<double, 64, 0>(a, wpb, s);
+    return hipErrorInvalidValue;
 }
 
-template <int LPF, int N2>
+template <int LPF, int BPL, int N2>
 static hipError_t launch_shape_t(const BondArgs& a, int wpb, hipStream_t s)
 {
-    const int fpw = 64 / LPF;
-    const int grid = ceil_div(a.n, (long)wpb * fpw);
+    const int grid = ceil_div(a.n, (long)wpb * Tile<LPF, BPL>::FPW);
     if (grid == 0) return hipSuccess;
-    const size_t lds = (size_t)wpb * 64 * 8 * sizeof(double);
-    hipLaunchKernelGGL((k_shape<LPF, N2>), dim3(grid), dim3(64 * wpb), lds, s, a);
+    const size_t lds = (size_t)wpb * Tile<LPF, BPL>::REC * 8 * sizeof(double);
+    hipLaunchKernelGGL((k_shape<LPF, BPL, N2>), dim3(grid), dim3(64 * wpb), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_shape(const BondArgs& a, int lpf, int n2_is_10, int wpb, hipStream_t s)
+hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int wpb, hipStream_t s)
 {
-    if (lpf == 32) return n2_is_10 ? launch_shape_t<32, 1>(a, wpb, s) : launch_shape_t<32, 0>(a, wpb, s);
-    return n2_is_10 ? launch_shape_t<64, 1>(a, wpb, s) : launch_shape_t<64, 0>(a, wpb, s);
+    if (bpl == 1 && lpf == 32) return n2_is_10 ? launch_shape_t<32, 1, 1>(a, wpb, s) : launch_shape_t<32, 1, 0>(a, wpb, s);
+    if (bpl == 1 && lpf == 64) return n2_is_10 ? launch_shape_t<64, 1, 1>(a, wpb, s) : launch_shape_t<64, 1, 0>(a, wpb, s);
+    if (bpl == 2 && lpf == 16) return n2_is_10 ? launch_shape_t<16, 2, 1>(a, wpb, s) : launch_shape_t<16, 2, 0>(a, wpb, s);
+    if (bpl == 2 && lpf == 32) return n2_is_10 ? launch_shape_t<32, 2, 1>(a, wpb, s) : launch_shape_t<32, 2, 0>(a, wpb, s);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_gather(const GatherArgs& g, int mixed, hipStream_t s)

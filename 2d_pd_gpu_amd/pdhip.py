@@ -33,7 +33,7 @@ OPT_USE_GRAPH = 1
 OPT_PROFILE = 2
 OPT_WAVES_PER_WG = 3
 OPT_XCD_REMAP = 4
-OPT_LANES_PER_FAM = 5
+OPT_TILE = 5
 
 # every entry point include/pd_hip.h declares
 EXPORTS = (
@@ -72,7 +72,8 @@ class Params(C.Structure):
 class Stats(C.Structure):
     _fields_ = [
         ("n", C.c_int), ("nb", C.c_long), ("n_broken", C.c_long), ("n_singular", C.c_long),
-        ("max_family", C.c_int), ("lanes_per_family", C.c_int), ("steps", C.c_long),
+        ("max_family", C.c_int), ("lanes_per_family", C.c_int), ("bonds_per_lane", C.c_int),
+        ("steps", C.c_long),
         ("phase_ms", C.c_double * 4), ("phase_calls", C.c_long * 4),
     ]
 

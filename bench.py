@@ -92,9 +92,14 @@ def read_traffic(workload, prec):
         return json.load(fh).get("bytes_per_launch")
 
 
-def run(pkg, w, precision, steps, warmup):
+def run(pkg, w, precision, steps, warmup, tune):
     pdh = pkg.pdhip
     ctx = w.context(precision=precision)
+    if tune:
+        # pick the family tile / workgroup shape on this plate (state restored)
+        t0 = time.perf_counter()
+        ctx.autotune(tune)
+        print(f"# autotune {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     ctx.step(warmup)
     ctx.synchronize()
     # timed region 1: graph replay (the production path)
@@ -121,6 +126,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="plate_1024")
+    ap.add_argument("--tune", type=int, default=3, help="autotuner steps per variant (0: off)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
@@ -139,7 +145,7 @@ def main():
     for name, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
         if name == "mixed" and args.no_mixed:
             continue
-        t_graph, t_prof, st, state = run(pkg, w, prec, args.steps, args.warmup)
+        t_graph, t_prof, st, state = run(pkg, w, prec, args.steps, args.warmup, args.tune)
         nb_alive = w.nb - st.n_broken
         calls = max(1, st.phase_calls[1])
         bond_ms = st.phase_ms[1] / calls
@@ -148,7 +154,7 @@ def main():
         kb, sb = kernel_bytes(w.n, w.nb, prec == pkg.MIXED)
         res[name] = dict(t=t_graph, t_prof=t_prof, nb_alive=nb_alive, bond_ms=bond_ms,
                          gather_ms=gather_ms, pos_ms=pos_ms, kb=kb, sb=sb, state=state,
-                         n_broken=st.n_broken)
+                         n_broken=st.n_broken, tile=(st.lanes_per_family, st.bonds_per_lane))
         print(f"# {name}: {args.steps} steps graph {t_graph:.3f}s eager+events {t_prof:.3f}s "
               f"bond {bond_ms:.4f} ms gather {gather_ms:.4f} ms positions {pos_ms:.4f} ms "
               f"broken {st.n_broken}", file=sys.stderr, flush=True)
@@ -175,6 +181,7 @@ def main():
         "particle_steps_per_s": w.n * args.steps / r["t"],
         "step_hbm_frac": r["sb"] / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "kernel_ms": {"bond": r["bond_ms"], "gather": r["gather_ms"], "positions": r["pos_ms"]},
+        "tile": {"lanes_per_family": r["tile"][0], "bonds_per_lane": r["tile"][1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "kernel": "k_bond<double>",
                      "traffic": read_traffic(w.name, "fp64")},
@@ -194,6 +201,7 @@ def main():
             "particle_steps_per_s": w.n * args.steps / m["t"],
             "step_hbm_frac": m["sb"] / (ms_m * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "kernel_ms": {"bond": m["bond_ms"], "gather": m["gather_ms"], "positions": m["pos_ms"]},
+            "tile": {"lanes_per_family": m["tile"][0], "bonds_per_lane": m["tile"][1]},
             "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach_m / HBM_PEAK_GBS, "kernel": "k_bond<float>",
                          "traffic": read_traffic(w.name, "mixed")},

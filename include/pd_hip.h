@@ -60,7 +60,9 @@ extern "C" {
 #define PD_OPT_PROFILE       2  /* 1: time each kernel phase with HIP events          */
 #define PD_OPT_WAVES_PER_WG  3  /* families-per-wavefront tile: waves per workgroup   */
 #define PD_OPT_XCD_REMAP     4  /* 1: map consecutive families to one XCD (default 1) */
-#define PD_OPT_LANES_PER_FAM 5  /* 32 or 64 lanes per family (0 = by max family size)  */
+#define PD_OPT_TILE          5  /* family tile: 100*lanes_per_family + bonds_per_lane:
+                                   3201, 6401, 1602 (and 3202 in mixed precision);
+                                   0 = 3201 or 6401 by the largest family            */
 
 typedef struct pd_params {
     int    ndim;       /* must be 2 */
@@ -83,6 +85,7 @@ typedef struct pd_stats {
     long   n_singular;     /* bonds whose K was singular at the last assembly */
     int    max_family;     /* largest family */
     int    lanes_per_family;
+    int    bonds_per_lane;
     long   steps;          /* steps run by pd_step since creation */
     double phase_ms[4];    /* profiled time: positions, bonds, gather, other */
     long   phase_calls[4];

@@ -263,22 +263,51 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     say("launch-shape variants and autotune")
     pdh = pdpkg.pdhip
     ref, bref, _ = run_plate_gpu(pdh, small_plate, 0.005, 40)
-    for wpb in (1, 2, 8):
-        for xcd in (0, 1):
-            for lpf in (32, 64):
+    mask = small_plate.code >= 0
+    for tile in (3201, 6401, 1602):
+        tile_ref = None
+        for wpb in (1, 2, 8):
+            for xcd in (0, 1):
                 ctx = small_plate.context(s0=0.005)
                 ctx.set_option(pdh.OPT_WAVES_PER_WG, wpb)
                 ctx.set_option(pdh.OPT_XCD_REMAP, xcd)
-                ctx.set_option(pdh.OPT_LANES_PER_FAM, lpf)
+                ctx.set_option(pdh.OPT_TILE, tile)
                 ctx.step(40)
-                st = ctx.get_state()
-                np.testing.assert_array_equal(st["v"], ref["v"])
+                v = ctx.get_state()["v"]
                 np.testing.assert_array_equal(ctx.get_broken(), bref)
                 ctx.close()
+                # the workgroup shape and XCD mapping never change the bits
+                if tile_ref is None:
+                    tile_ref = v
+                np.testing.assert_array_equal(v, tile_ref)
+        # two bonds per lane reorder the FP64 family sums: round-off only
+        _, nonbc, _ = H.parity_norms(tile_ref, ref["v"], mask)
+        assert nonbc < 1e-13
+    mref = None
+    for tile in (3201, 6401, 1602, 3202):
+        ctx = small_plate.context(precision=1, s0=0.005)
+        ctx.set_option(pdh.OPT_TILE, tile)
+        ctx.step(40)
+        v = ctx.get_state()["v"]
+        ctx.close()
+        if mref is None:
+            mref = v
+        _, nonbc, _ = H.parity_norms(v, mref, mask)
+        assert nonbc < 1e-6
+    with pytest.raises(pdh.PDError):
+        ctx = small_plate.context(s0=0.005)
+        try:
+            ctx.set_option(pdh.OPT_TILE, 3202)  # FP64 has no (32, 2) tile
+        finally:
+            ctx.close()
     ctx = small_plate.context(s0=0.005)
     ctx.autotune(3)
+    st = ctx.stats()
+    assert st.steps == 0  # the autotuner restores the state it timed on
     ctx.step(40)
-    np.testing.assert_array_equal(ctx.get_state()["v"], ref["v"])
+    _, nonbc, _ = H.parity_norms(ctx.get_state()["v"], ref["v"], mask)
+    assert nonbc < 1e-13
+    np.testing.assert_array_equal(ctx.get_broken(), bref)
     ctx.close()
 
 
@@ -297,7 +326,7 @@ def test_delta_4015_family_of_48(pdpkg):
     assert np.diff(w.row).max() == 48
     u, v, f, br = run_plate_oracle(w, 0.0, 30)
     st, _, stats = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30)
-    assert stats.lanes_per_family == 64
+    assert stats.lanes_per_family * stats.bonds_per_lane >= 48
     _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
     assert nonbc < 1e-11
     st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, precision=1)
