@@ -11,11 +11,11 @@
 // they do not depend on j, and w is symmetric, so each family is evaluated once
 // here (O(M^2)).  Displacement form keeps F - I free of cancellation.
 //
-// Layout: one family per half-wave (families of <= 32 bonds, 2 per wave) or per
-// wave (<= 64 bonds).  Lane k owns bond k of its family.  Each lane keeps its
-// weight row w(k, .) in registers; the per-bond operands that every lane needs
-// (unit bond vector, radial factors, V du (x) xi, later V sigma K^-1) go through
-// a per-wave LDS record that the l-loop reads as a broadcast.  No float
+// Layout: a family of up to FS = LPF * BPL bonds is handled by LPF lanes, each
+// owning BPL bonds; a wave holds 64 / LPF families.  Each lane keeps its weight
+// rows w(k, .) in registers; the per-bond operands that every lane of a family
+// needs (unit bond vector, radial factors, V du (x) xi, later V sigma K^-1) go
+// through per-wave LDS records that the l-loop reads as broadcasts.  No float
 // atomics: each family writes its own-bond sum and, into the reverse-bond slot
 // of each neighbour's row, the term that neighbour must subtract; the gather
 // kernel then reads one contiguous row per particle.
@@ -54,37 +54,33 @@ __device__ __forceinline__ double bond_stretch(double x0, double x1, double d0, 
     return (cur - len) / len;
 }
 
-template <typename R> __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
-template <typename R> __device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+__device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double rmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double rpow(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float rpow(float a, float b) { return powf(a, b); }
 
-// w(k,l) from unit bond vectors and the radial factors p = exp(A l), q = exp(-A l):
-// exp(-A |l_k - l_l|) = min(q_k p_l, p_k q_l).
+// w(k,l) = exp(-A |l_k - l_l|) (1/2 + 1/2 cos theta_kl)^n2 from the k-side half
+// unit vector (hx, hy) = e_k / 2, the l-side unit vector, and the radial
+// factors p = exp(A l), q = exp(-A l): exp(-A |l_k - l_l|) = min(q_k p_l, p_k q_l).
+// 1/2 + 1/2 cos lies in [0, 1] up to rounding; for n2 = 10 the power is four
+// squarings and needs no clamp (a value a few ulp outside moves w by a few ulp),
+// the general power clamps at 0 (src/main_seq.cpp:115 clamps cos to [-1, 1]).
 template <typename R, int N2_IS_10>
-__device__ __forceinline__ R weight(R exk, R eyk, R pk, R qk, R exl, R eyl, R pl, R ql, R n2)
+__device__ __forceinline__ R weight(R hxk, R hyk, R pk, R qk, R exl, R eyl, R pl, R ql, R n2)
 {
-    R c = exk * exl + eyk * eyl;
-    R h = R(0.5) + R(0.5) * c;
-    h = rmin(rmax(h, R(0)), R(1));
+    const R h = hxk * exl + (hyk * eyl + R(0.5));
     R hp;
     if constexpr (N2_IS_10) {
-        R h2 = h * h;
-        R h4 = h2 * h2;
-        R h8 = h4 * h4;
+        const R h2 = h * h;
+        const R h4 = h2 * h2;
+        const R h8 = h4 * h4;
         hp = h8 * h2;
     } else {
-        hp = pow(h, n2);
+        hp = rpow(rmax(h, R(0)), n2);
     }
     return rmin(qk * pl, pk * ql) * hp;
-}
-
-template <> __device__ __forceinline__ float weight<float, 0>(float exk, float eyk, float pk, float qk,
-                                                              float exl, float eyl, float pl, float ql,
-                                                              float n2)
-{
-    float c = exk * exl + eyk * eyl;
-    float h = 0.5f + 0.5f * c;
-    h = rmin(rmax(h, 0.0f), 1.0f);
-    return rmin(qk * pl, pk * ql) * powf(h, n2);
 }
 
 // Sum over the lanes of one family (LPF lanes, aligned).
@@ -110,6 +106,18 @@ template <int LPF, int BPL> struct Tile {
     static constexpr int FPW = 64 / LPF;
     static constexpr int REC = 64 * BPL;  // LDS records (8 values each) per wave
 };
+
+// LDS record layout.  Value e (0..7) of record k of family `slot` sits in
+// 16-byte chunk e / VPC of that record; chunks are interleaved across the
+// wave's FPW families, so one broadcast read (every lane of a family asks for
+// the same chunk) touches FPW consecutive 16-byte words in distinct banks.
+template <typename T, int FPW>
+__device__ __forceinline__ int rec_at(int k, int slot, int e)
+{
+    constexpr int VPC = 16 / (int)sizeof(T);
+    constexpr int NCH = 8 / VPC;
+    return ((k * NCH + e / VPC) * FPW + slot) * VPC + e % VPC;
+}
 
 // The family a lane works on.
 struct LaneFam {
@@ -144,7 +152,7 @@ __device__ __forceinline__ LaneFam load_family(const BondArgs& a, int lane, int 
 struct BondSlot {
     int b, k;
     bool act;
-    double x0, x1, d0, d1, len, Vj;
+    double x0, x1, d0, d1, Vj;
 };
 
 __device__ __forceinline__ BondSlot load_slot(const BondArgs& a, const LaneFam& F, int k)
@@ -159,63 +167,96 @@ __device__ __forceinline__ BondSlot load_slot(const BondArgs& a, const LaneFam& 
     S.x1 = Xj.y - F.Xi.y;
     S.d0 = uj.x - F.ui.x;
     S.d1 = uj.y - F.ui.y;
-    S.len = sqrt(S.x0 * S.x0 + S.x1 * S.x1);
     S.Vj = a.V[j];
     return S;
 }
 
-// Bond geometry for the weight: unit vector and radial factors (FP64).
-struct Geo {
-    double ex, ey, pe, qe;
+// Bond geometry for the weight: unit vector e, half vector e/2 and the radial
+// factors exp(+-A l).  Inactive slots get all zeros, so their weight is 0.
+template <typename R> struct Geo {
+    R ex, ey, hx, hy, pe, qe;
 };
 
-__device__ __forceinline__ Geo geometry(const BondArgs& a, const BondSlot& S)
+__device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondSlot& S)
 {
-    Geo g;
-    const double il = S.act ? 1.0 / S.len : 0.0;
+    Geo<double> g;
+    const double len = sqrt(S.x0 * S.x0 + S.x1 * S.x1);
+    const double il = S.act ? 1.0 / len : 0.0;
     g.ex = S.x0 * il;
     g.ey = S.x1 * il;
-    g.pe = S.act ? exp(a.A * S.len) : 0.0;
-    g.qe = S.act ? exp(-a.A * S.len) : 0.0;
+    g.hx = 0.5 * g.ex;
+    g.hy = 0.5 * g.ey;
+    g.pe = S.act ? exp(a.A * len) : 0.0;
+    g.qe = S.act ? exp(-a.A * len) : 0.0;
     return g;
 }
+
+// FP32 geometry for the mixed-precision contractions (hardware exp / rcp).
+__device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSlot& S)
+{
+    Geo<float> g;
+    const float x0 = float(S.x0), x1 = float(S.x1);
+    const float len = sqrtf(x0 * x0 + x1 * x1);
+    const float il = S.act ? __builtin_amdgcn_rcpf(len) : 0.0f;
+    g.ex = x0 * il;
+    g.ey = x1 * il;
+    g.hx = 0.5f * g.ex;
+    g.hy = 0.5f * g.ey;
+    const float Al = float(a.A) * len;
+    g.pe = S.act ? __expf(Al) : 0.0f;
+    g.qe = S.act ? __expf(-Al) : 0.0f;
+    return g;
+}
+
+template <typename R> struct GeoOf;
+template <> struct GeoOf<double> {
+    static __device__ __forceinline__ Geo<double> get(const BondArgs& a, const BondSlot& S) { return geometry64(a, S); }
+};
+template <> struct GeoOf<float> {
+    static __device__ __forceinline__ Geo<float> get(const BondArgs& a, const BondSlot& S) { return geometry32(a, S); }
+};
 
 // FP64 assembly of K_ik over the alive bonds of each family in the wave, and
 // its inverse under the scale-relative singularity rule.  rec: this wave's LDS
 // records (Tile::REC x 8 doubles).  Every lane of the wave must call it.
 template <int LPF, int BPL, int N2_IS_10>
 __device__ __forceinline__ void assemble_kinv(double* rec, const LaneFam& F, const BondSlot* S,
-                                              const bool* alive, const Geo* g, double n2,
+                                              const bool* alive, const Geo<double>* g, double n2,
                                               double ktol, double (*kinv)[3], bool* singular)
 {
     constexpr int FS = Tile<LPF, BPL>::FS;
+    constexpr int FPW = Tile<LPF, BPL>::FPW;
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
-        double* my = rec + (F.slot * FS + S[s].k) * 8;
+        const int k = S[s].k;
         const double V = alive[s] ? S[s].Vj : 0.0;
-        my[0] = g[s].ex;
-        my[1] = g[s].ey;
-        my[2] = g[s].pe;
-        my[3] = g[s].qe;
-        my[4] = V * S[s].x0 * S[s].x0;
-        my[5] = V * S[s].x0 * S[s].x1;
-        my[6] = V * S[s].x1 * S[s].x1;
+        rec[rec_at<double, FPW>(k, F.slot, 0)] = g[s].ex;
+        rec[rec_at<double, FPW>(k, F.slot, 1)] = g[s].ey;
+        rec[rec_at<double, FPW>(k, F.slot, 2)] = g[s].pe;
+        rec[rec_at<double, FPW>(k, F.slot, 3)] = g[s].qe;
+        rec[rec_at<double, FPW>(k, F.slot, 4)] = V * S[s].x0 * S[s].x0;
+        rec[rec_at<double, FPW>(k, F.slot, 5)] = V * S[s].x0 * S[s].x1;
+        rec[rec_at<double, FPW>(k, F.slot, 6)] = V * S[s].x1 * S[s].x1;
     }
     wave_sync();
-    const double* base = rec + F.slot * FS * 8;
     double K[BPL][3];
 #pragma unroll
     for (int s = 0; s < BPL; ++s) K[s][0] = K[s][1] = K[s][2] = 0.0;
 #pragma unroll 2
     for (int l = 0; l < FS; ++l) {
-        const double* r = base + l * 8;
+        const double ex = rec[rec_at<double, FPW>(l, F.slot, 0)];
+        const double ey = rec[rec_at<double, FPW>(l, F.slot, 1)];
+        const double pe = rec[rec_at<double, FPW>(l, F.slot, 2)];
+        const double qe = rec[rec_at<double, FPW>(l, F.slot, 3)];
+        const double kxx = rec[rec_at<double, FPW>(l, F.slot, 4)];
+        const double kxy = rec[rec_at<double, FPW>(l, F.slot, 5)];
+        const double kyy = rec[rec_at<double, FPW>(l, F.slot, 6)];
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
-            double w = weight<double, N2_IS_10>(g[s].ex, g[s].ey, g[s].pe, g[s].qe, r[0], r[1], r[2],
-                                                r[3], n2);
-            K[s][0] += w * r[4];
-            K[s][1] += w * r[5];
-            K[s][2] += w * r[6];
+            const double w = weight<double, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2);
+            K[s][0] += w * kxx;
+            K[s][1] += w * kxy;
+            K[s][2] += w * kyy;
         }
     }
     wave_sync();
@@ -247,7 +288,7 @@ __device__ __forceinline__ int wave_max_family(int m)
 template <int LPF, int BPL, int N2_IS_10>
 __global__ void __launch_bounds__(512) k_shape(BondArgs a)
 {
-    extern __shared__ double smem[];
+    extern __shared__ __attribute__((aligned(16))) double smem[];
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
@@ -255,12 +296,12 @@ __global__ void __launch_bounds__(512) k_shape(BondArgs a)
     const LaneFam F = load_family<LPF>(a, lane, blk * wpb + wib);
     BondSlot S[BPL];
     bool alive[BPL];
-    Geo g[BPL];
+    Geo<double> g[BPL];
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         S[s] = load_slot(a, F, F.t + s * LPF);
         alive[s] = S[s].act && a.broken[S[s].b] == 0;
-        g[s] = geometry(a, S[s]);
+        g[s] = geometry64(a, S[s]);
     }
     double kinv[BPL][3];
     bool sing[BPL];
@@ -286,7 +327,8 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
 {
     typedef typename Vec2<R>::T R2;
     constexpr int FS = Tile<LPF, BPL>::FS;
-    extern __shared__ double smem[];
+    constexpr int FPW = Tile<LPF, BPL>::FPW;
+    extern __shared__ __attribute__((aligned(16))) double smem[];
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
@@ -313,17 +355,16 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
         }
         nnew += __popcll(__ballot(newly));
     }
-    const int nm = nnew;
     if (lane == 0 && nnew) atomicAdd(a.counters, (unsigned int)nnew);
-    Geo g[BPL];
-#pragma unroll
-    for (int s = 0; s < BPL; ++s) g[s] = geometry(a, S[s]);
 
     // ---- K_ik^-1: stored, or re-assembled in FP64 when the family changed ----
     double kinv[BPL][3];
-    if (nm != 0 || a.force_k) {
+    if (nnew != 0 || a.force_k) {
+        Geo<double> g64[BPL];
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) g64[s] = geometry64(a, S[s]);
         bool sing[BPL];
-        assemble_kinv<LPF, BPL, N2_IS_10>(rec64, F, S, alive, g, a.n2, a.ktol, kinv, sing);
+        assemble_kinv<LPF, BPL, N2_IS_10>(rec64, F, S, alive, g64, a.n2, a.ktol, kinv, sing);
         int nsing = 0;
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
@@ -348,57 +389,67 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     }
 
     // ---- records for the first contraction ----
-    R* rec = reinterpret_cast<R*>(rec64);
-    R gx[BPL], gy[BPL], gp[BPL], gq[BPL];
+    R* rec = reinterpret_cast<R*>(__builtin_assume_aligned(rec64, 16));
+    Geo<R> g[BPL];
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
-        gx[s] = R(g[s].ex);
-        gy[s] = R(g[s].ey);
-        gp[s] = R(g[s].pe);
-        gq[s] = R(g[s].qe);
+        g[s] = GeoOf<R>::get(a, S[s]);
         const R x0r = R(S[s].x0), x1r = R(S[s].x1);
         const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) : R(0);
         const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) : R(0);
-        R* my = rec + (F.slot * FS + S[s].k) * 8;
-        my[0] = gx[s];
-        my[1] = gy[s];
-        my[2] = gp[s];
-        my[3] = gq[s];
-        my[4] = Vd0 * x0r;
-        my[5] = Vd0 * x1r;
-        my[6] = Vd1 * x0r;
-        my[7] = Vd1 * x1r;
+        const int k = S[s].k;
+        rec[rec_at<R, FPW>(k, F.slot, 0)] = g[s].ex;
+        rec[rec_at<R, FPW>(k, F.slot, 1)] = g[s].ey;
+        rec[rec_at<R, FPW>(k, F.slot, 2)] = g[s].pe;
+        rec[rec_at<R, FPW>(k, F.slot, 3)] = g[s].qe;
+        rec[rec_at<R, FPW>(k, F.slot, 4)] = Vd0 * x0r;
+        rec[rec_at<R, FPW>(k, F.slot, 5)] = Vd0 * x1r;
+        rec[rec_at<R, FPW>(k, F.slot, 6)] = Vd1 * x0r;
+        rec[rec_at<R, FPW>(k, F.slot, 7)] = Vd1 * x1r;
     }
+    // records past a family's end are zero (inactive slots), so the loops may
+    // run to the wave's largest family rounded up to 4 without guards
     const int mmax = wave_max_family<LPF>(F.M);
     wave_sync();
 
     // ---- contraction 1: D_ik = sum_l w(k,l) V_l du_l (x) xi_l ----
-    const R* base = rec + F.slot * FS * 8;
     const R n2r = R(a.n2);
     R w[BPL][FS];
     R D[BPL][4];
 #pragma unroll
     for (int s = 0; s < BPL; ++s) D[s][0] = D[s][1] = D[s][2] = D[s][3] = R(0);
 #pragma unroll
-    for (int l = 0; l < FS; ++l) {
-        if (l < mmax) {
-            const R* r = base + l * 8;
+    for (int l0 = 0; l0 < FS; l0 += 4) {
+        if (l0 < mmax) {
 #pragma unroll
-            for (int s = 0; s < BPL; ++s) {
-                const R om = weight<R, N2_IS_10>(gx[s], gy[s], gp[s], gq[s], r[0], r[1], r[2], r[3], n2r);
-                w[s][l] = om;
-                D[s][0] += om * r[4];
-                D[s][1] += om * r[5];
-                D[s][2] += om * r[6];
-                D[s][3] += om * r[7];
+            for (int l = l0; l < l0 + 4; ++l) {
+                const R ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
+                const R ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
+                const R pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
+                const R qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
+                const R P0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
+                const R P1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
+                const R P2 = rec[rec_at<R, FPW>(l, F.slot, 6)];
+                const R P3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
+#pragma unroll
+                for (int s = 0; s < BPL; ++s) {
+                    const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                    w[s][l] = om;
+                    D[s][0] += om * P0;
+                    D[s][1] += om * P1;
+                    D[s][2] += om * P2;
+                    D[s][3] += om * P3;
+                }
             }
         } else {
 #pragma unroll
-            for (int s = 0; s < BPL; ++s) w[s][l] = R(0);
+            for (int l = l0; l < l0 + 4; ++l)
+#pragma unroll
+                for (int s = 0; s < BPL; ++s) w[s][l] = R(0);
         }
         // keep the scheduler from hoisting every record load of the unrolled
         // loop (which would spill)
-        if ((l & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
     }
 
     // ---- F - I = D K^-1, plane-stress sigma, Q_k = V_k sigma K^-1 ----
@@ -426,11 +477,11 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     wave_sync();
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
-        R* my = rec + (F.slot * FS + S[s].k) * 8;
-        my[4] = Q[s][0];
-        my[5] = Q[s][1];
-        my[6] = Q[s][2];
-        my[7] = Q[s][3];
+        const int k = S[s].k;
+        rec[rec_at<R, FPW>(k, F.slot, 4)] = Q[s][0];
+        rec[rec_at<R, FPW>(k, F.slot, 5)] = Q[s][1];
+        rec[rec_at<R, FPW>(k, F.slot, 6)] = Q[s][2];
+        rec[rec_at<R, FPW>(k, F.slot, 7)] = Q[s][3];
     }
     wave_sync();
 
@@ -439,18 +490,24 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
 #pragma unroll
     for (int s = 0; s < BPL; ++s) Sm[s][0] = Sm[s][1] = Sm[s][2] = Sm[s][3] = R(0);
 #pragma unroll
-    for (int l = 0; l < FS; ++l) {
-        if (l < mmax) {
-            const R* r = base + l * 8;
+    for (int l0 = 0; l0 < FS; l0 += 4) {
+        if (l0 < mmax) {
 #pragma unroll
-            for (int s = 0; s < BPL; ++s) {
-                Sm[s][0] += w[s][l] * r[4];
-                Sm[s][1] += w[s][l] * r[5];
-                Sm[s][2] += w[s][l] * r[6];
-                Sm[s][3] += w[s][l] * r[7];
+            for (int l = l0; l < l0 + 4; ++l) {
+                const R Q0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
+                const R Q1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
+                const R Q2 = rec[rec_at<R, FPW>(l, F.slot, 6)];
+                const R Q3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
+#pragma unroll
+                for (int s = 0; s < BPL; ++s) {
+                    Sm[s][0] += w[s][l] * Q0;
+                    Sm[s][1] += w[s][l] * Q1;
+                    Sm[s][2] += w[s][l] * Q2;
+                    Sm[s][3] += w[s][l] * Q3;
+                }
             }
         }
-        if ((l & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_barrier(0);
     }
     double vsum = 0.0;
 #pragma unroll
@@ -488,21 +545,35 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
 }
 
 // f_i = sum_j T_ij V_j - sum_j T_ji V_i ; v += 1/2 (a + f/rho) dt.
+// GATHER_LANES lanes per particle read its W row with unit stride (coalesced),
+// then reduce across the group.
+constexpr int GATHER_LANES = 8;
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_gather(GatherArgs g)
 {
     typedef typename Vec2<R>::T R2;
     const int blk = xcd_remap(blockIdx.x, gridDim.x, g.xcd);
-    const int i = blk * blockDim.x + threadIdx.x;
-    if (i >= g.n) return;
+    const int gid = blk * blockDim.x + threadIdx.x;
+    const int i = gid / GATHER_LANES;
+    const int t = gid % GATHER_LANES;
+    const bool ok = i < g.n;
     const R2* W = reinterpret_cast<const R2*>(g.W);
     double f0 = 0.0, f1 = 0.0;
-    const int e = g.row[i + 1];
-    for (int b = g.row[i]; b < e; ++b) {
-        const R2 t = W[b];
-        f0 += double(t.x);
-        f1 += double(t.y);
+    if (ok) {
+        const int e = g.row[i + 1];
+        for (int b = g.row[i] + t; b < e; b += GATHER_LANES) {
+            const R2 w = W[b];
+            f0 += double(w.x);
+            f1 += double(w.y);
+        }
     }
+#pragma unroll
+    for (int off = GATHER_LANES / 2; off > 0; off >>= 1) {
+        f0 += __shfl_xor(f0, off, 64);
+        f1 += __shfl_xor(f1, off, 64);
+    }
+    if (!ok || t != 0) return;
     const double2 fo = g.fown[i];
     f0 = fo.x - f0;
     f1 = fo.y - f1;
@@ -630,7 +701,7 @@ hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int w
 
 hipError_t launch_gather(const GatherArgs& g, int mixed, hipStream_t s)
 {
-    const int grid = ceil_div(g.n, 256);
+    const int grid = ceil_div((long)g.n * GATHER_LANES, 256);
     if (grid == 0) return hipSuccess;
     if (mixed)
         hipLaunchKernelGGL(k_gather<float>, dim3(grid), dim3(256), 0, s, g);
