@@ -56,6 +56,8 @@ struct pd_context {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int use_graph = 1, profile = 0, wpb = 4, xcd = 8, lpf_opt = 0;
+    int persist = 0;  // persistent bond kernel: workgroups per CU (0 = one wave per group)
+    int n_cu = 256;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
@@ -162,8 +164,10 @@ static BondArgs bond_args(pd_context* c)
     a.kinv = c->kinv;
     a.W = c->W;
     a.fown = c->fown;
-    a.E = c->E;
-    a.E0 = c->p.E;
+    // plane-stress factor E/(1-nu^2) of getStiffnessTensor (src/main_seq.cpp:185)
+    a.pf = c->E;
+    a.pf0 = c->p.E / (1.0 - c->p.nu * c->p.nu);
+    a.brk_c = (1.0 + c->p.s0) * (1.0 + c->p.s0);
     a.nu = c->p.nu;
     a.A = c->p.n1 / (c->p.horizon * c->p.dx);
     a.n2 = c->p.n2;
@@ -413,6 +417,9 @@ extern "C" int pd_create(const pd_params* p, int n, const double* X, const doubl
         delete c;
         return fail(nullptr, PD_ERR_HIP, "no HIP device available");
     }
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
+        c->n_cu <= 0)
+        c->n_cu = 256;
     int rc = PD_OK;
     const size_t wbytes = c->p.precision == PD_MIXED ? 8 : 16;
     if ((rc = dalloc(c, &c->X, n)) || (rc = dalloc(c, &c->u, n)) || (rc = dalloc(c, &c->v, n)) ||
@@ -535,6 +542,10 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
         c->wpb = (int)value;
         break;
     case PD_OPT_XCD_REMAP: c->xcd = value ? 8 : 1; break;
+    case PD_OPT_PERSISTENT:
+        if (value < 0 || value > 64) return fail(c, PD_ERR_INVALID, "persistent workgroups per CU must be in [0, 64]");
+        c->persist = (int)value;
+        break;
     case PD_OPT_TILE: {
         int lpf = (int)(value / 100), bpl = (int)(value % 100);
         if (value == 0) {
@@ -557,10 +568,12 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
 extern "C" int pd_set_youngs_modulus(pd_context* c, const double* E)
 {
     if (!c || !E) return fail(c, PD_ERR_INVALID, "NULL argument");
+    // stored as the plane-stress factor E/(1-nu^2) (src/main_seq.cpp:185)
     std::vector<double> Ei(c->n);
     for (int q = 0; q < c->n; ++q) {
-        Ei[q] = E[c->perm[q]];
-        if (!(Ei[q] > 0)) return fail(c, PD_ERR_INVALID, "E[%d] must be positive", c->perm[q]);
+        const double e = E[c->perm[q]];
+        if (!(e > 0)) return fail(c, PD_ERR_INVALID, "E[%d] must be positive", c->perm[q]);
+        Ei[q] = e / (1.0 - c->p.nu * c->p.nu);
     }
     if (!c->E)
         if (int rc = dalloc(c, &c->E, c->n)) return rc;
@@ -674,7 +687,7 @@ static int force_and_velocity(pd_context* c, int write_a)
     BondArgs a = bond_args(c);
     GatherArgs g = gather_args(c, write_a);
     int mixed = c->p.precision == PD_MIXED;
-    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->n2_is_10, c->wpb, c->stream); }))
+    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->n2_is_10, c->wpb, c->persist * c->n_cu, c->stream); }))
         return rc;
     return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream); });
 }
@@ -817,6 +830,8 @@ extern "C" int pd_get_stats(pd_context* c, pd_stats* s)
     s->max_family = c->maxM;
     s->lanes_per_family = c->lpf;
     s->bonds_per_lane = c->bpl;
+    s->persistent = c->persist;
+    s->waves_per_wg = c->wpb;
     s->steps = c->steps;
     s->n_singular = c->n_singular;
     std::vector<unsigned char> t(c->nb);
@@ -889,22 +904,25 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     const int saved_profile = c->profile;
     const long saved_steps = c->steps;
     c->profile = 0;
-    struct Var { int wpb, xcd, lpf, bpl; };
+    struct Var { int wpb, xcd, lpf, bpl, persist; };
     std::vector<Var> vars;
     const int mixed = c->p.precision == PD_MIXED;
     const int tiles[4][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}};
     for (const auto& tl : tiles) {
         if (!tile_supported(mixed, tl[0], tl[1]) || tl[0] * tl[1] < c->maxM) continue;
         for (int w : {1, 2, 4, 8})
-            for (int x : {1, 8}) vars.push_back({w, x, tl[0], tl[1]});
+            for (int x : {1, 8})
+                for (int pz : {0, 1, 2, 4})
+                    if (pz == 0 || w * pz <= 32) vars.push_back({w, x, tl[0], tl[1], pz});
     }
     double best = 1e300;
-    Var bestv = {c->wpb, c->xcd, c->lpf, c->bpl};
+    Var bestv = {c->wpb, c->xcd, c->lpf, c->bpl, c->persist};
     for (const Var& vv : vars) {
         c->wpb = vv.wpb;
         c->xcd = vv.xcd;
         c->lpf = vv.lpf;
         c->bpl = vv.bpl;
+        c->persist = vv.persist;
         c->graph_valid = false;
         if ((rc = pd_step(c, 1))) break;  // warm-up
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -926,6 +944,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     c->xcd = bestv.xcd;
     c->lpf = bestv.lpf;
     c->bpl = bestv.bpl;
+    c->persist = bestv.persist;
     c->graph_valid = false;
     c->profile = saved_profile;
     c->steps = saved_steps;
@@ -1045,4 +1064,49 @@ extern "C" long pd_notch_mask(int n, const double* X, const int* row_ptr, const 
         total += cnt;
     });
     return total;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: plain copy kernel of known traffic
+// ---------------------------------------------------------------------------
+extern "C" int pd_probe_copy(long bytes, int iters, double* gbps)
+{
+    if (bytes < 32 || iters <= 0 || !gbps) return fail(nullptr, PD_ERR_INVALID, "bad copy probe arguments");
+    const long n4 = bytes / 32;
+    double4 *src = nullptr, *dst = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PD_OK;
+    float ms = 0.f;
+    if (hipMalloc((void**)&src, 32 * n4) != hipSuccess || hipMalloc((void**)&dst, 32 * n4) != hipSuccess) {
+        rc = fail(nullptr, PD_ERR_NOMEM, "copy probe: hipMalloc failed");
+        goto done;
+    }
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(src, 0, 32 * n4, s) != hipSuccess ||
+        launch_copy_probe(src, dst, n4, s) != hipSuccess || hipEventRecord(e0, s) != hipSuccess) {
+        rc = fail(nullptr, PD_ERR_HIP, "copy probe: setup failed");
+        goto done;
+    }
+    for (int k = 0; k < iters; ++k)
+        if (launch_copy_probe(src, dst, n4, s) != hipSuccess) {
+            rc = fail(nullptr, PD_ERR_HIP, "copy probe: launch failed");
+            goto done;
+        }
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(nullptr, PD_ERR_HIP, "copy probe: timing failed");
+        goto done;
+    }
+    *gbps = 2.0 * 32.0 * (double)n4 * iters / (ms * 1e-3) / 1e9;
+done:
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) {
+        hipStreamSynchronize(s);
+        hipStreamDestroy(s);
+    }
+    if (src) hipFree(src);
+    if (dst) hipFree(dst);
+    return rc;
 }

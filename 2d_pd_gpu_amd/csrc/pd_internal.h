@@ -23,11 +23,12 @@ struct BondArgs {
     double* kinv;             // per bond K_ik^-1 (xx, xy, yy), FP64
     void* W;                  // per bond, slot rev[b] <- T_ij V_i (double2 / float2)
     double2* fown;            // per particle sum_j T_ij V_j
-    const double* E;          // per particle Young's modulus or nullptr
-    double E0, nu;            // uniform E, Poisson ratio
+    const double* pf;         // per particle plane-stress factor E/(1-nu^2), or nullptr
+    double pf0, nu;           // uniform E/(1-nu^2), Poisson ratio
     double A;                 // n1 / (horizon * dx)
     double n2;                // angular exponent
     double s0;                // critical stretch (<= 0: none)
+    double brk_c;             // (1 + s0)^2
     double ktol;              // singular-K tolerance
     int force_k;              // 1: assemble K for every family in this launch
     unsigned int* counters;   // [0] newly broken bonds, [1] singular K
@@ -60,8 +61,10 @@ struct PosArgs {
 // Tiles: (lanes per family, bonds per lane) in {(32,1), (64,1), (16,2)} and,
 // for mixed precision, (32,2); a family may hold up to lpf * bpl bonds.
 bool tile_supported(int mixed, int lpf, int bpl);
+// pgrid > 0: persistent launch with pgrid workgroups (each walks a contiguous
+// range of family groups, prefetching the next group's loads).
 hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int n2_is_10, int wpb,
-                       hipStream_t s);
+                       int pgrid, hipStream_t s);
 hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int wpb, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, int mixed, hipStream_t s);
 hipError_t launch_positions(const PosArgs& a, hipStream_t s);
@@ -69,5 +72,6 @@ hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double
                                    double2* a, hipStream_t s);
 hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
+hipError_t launch_copy_probe(const double4* src, double4* dst, long n4, hipStream_t s);
 
 }  // namespace pd

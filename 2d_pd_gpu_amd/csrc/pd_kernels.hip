@@ -42,16 +42,18 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks, int nxcd)
     return x * per + (x < rem ? x : rem) + idx;
 }
 
-// Bond stretch with every operation rounded on its own, so (i->j) and (j->i)
-// give the same bits; matches the oracle's or_bond_stretch.
-__device__ __forceinline__ double bond_stretch(double x0, double x1, double d0, double d1)
+// Breaking test s = (|xi + du| - |xi|) / |xi| > s0, evaluated as
+// |xi + du|^2 > c |xi|^2 with c = (1 + s0)^2 and every operation rounded on its
+// own, so (i->j) and (j->i) give the same bits; matches the oracle's
+// or_bond_breaks.
+__device__ __forceinline__ bool bond_breaks(double x0, double x1, double d0, double d1, double c)
 {
 #pragma clang fp contract(off)
-    double e0 = x0 + d0;
-    double e1 = x1 + d1;
-    double len = sqrt(x0 * x0 + x1 * x1);
-    double cur = sqrt(e0 * e0 + e1 * e1);
-    return (cur - len) / len;
+    const double e0 = x0 + d0;
+    const double e1 = x1 + d1;
+    const double cur2 = e0 * e0 + e1 * e1;
+    const double len2 = x0 * x0 + x1 * x1;
+    return cur2 > c * len2;
 }
 
 __device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
@@ -174,7 +176,7 @@ __device__ __forceinline__ BondSlot load_slot(const BondArgs& a, const LaneFam& 
 // Bond geometry for the weight: unit vector e, half vector e/2 and the radial
 // factors exp(+-A l).  Inactive slots get all zeros, so their weight is 0.
 template <typename R> struct Geo {
-    R ex, ey, hx, hy, pe, qe;
+    R ex, ey, hx, hy, pe, qe, len;
 };
 
 __device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondSlot& S)
@@ -182,6 +184,7 @@ __device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondS
     Geo<double> g;
     const double len = sqrt(S.x0 * S.x0 + S.x1 * S.x1);
     const double il = S.act ? 1.0 / len : 0.0;
+    g.len = len;
     g.ex = S.x0 * il;
     g.ey = S.x1 * il;
     g.hx = 0.5 * g.ex;
@@ -198,6 +201,7 @@ __device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSl
     const float x0 = float(S.x0), x1 = float(S.x1);
     const float len = sqrtf(x0 * x0 + x1 * x1);
     const float il = S.act ? __builtin_amdgcn_rcpf(len) : 0.0f;
+    g.len = len;
     g.ex = x0 * il;
     g.ey = x1 * il;
     g.hx = 0.5f * g.ex;
@@ -320,38 +324,84 @@ __global__ void __launch_bounds__(512) k_shape(BondArgs a)
     if (lane == 0 && nsing) atomicAdd(a.counters + 1, (unsigned int)nsing);
 }
 
-// One explicit force evaluation: bond breaking, K re-assembly for families that
-// lost a bond, then the two family contractions and the force-state outputs.
-template <typename R, int LPF, int BPL, int N2_IS_10>
-__global__ void __launch_bounds__(512) k_bond(BondArgs a)
+// Global inputs of one family group for one lane, gathered in three dependent
+// stages (row pointers -> neighbour ids -> neighbour state, broken flags and
+// K^-1), so that a persistent wave can issue each stage for its next group
+// while it computes the current one.
+template <int BPL> struct FamLoad {
+    LaneFam F;
+    BondSlot S[BPL];
+    int j[BPL];
+    unsigned char br[BPL];
+    double kinv[BPL][3];
+};
+
+template <int LPF, int BPL>
+__device__ __forceinline__ void stage_rows(const BondArgs& a, FamLoad<BPL>& L, int lane, int group)
+{
+    L.F = load_family<LPF>(a, lane, group);
+}
+
+template <int LPF, int BPL>
+__device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<BPL>& L)
+{
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        BondSlot& S = L.S[s];
+        S.k = L.F.t + s * LPF;
+        S.act = L.F.famok && S.k < L.F.M;
+        S.b = L.F.beg + S.k;
+        L.j[s] = S.act ? a.col[S.b] : (L.F.famok ? L.F.fam : 0);
+    }
+}
+
+template <int LPF, int BPL>
+__device__ __forceinline__ void stage_bonds(const BondArgs& a, FamLoad<BPL>& L)
+{
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        BondSlot& S = L.S[s];
+        const double2 Xj = a.X[L.j[s]], uj = a.u[L.j[s]];
+        S.x0 = Xj.x - L.F.Xi.x;
+        S.x1 = Xj.y - L.F.Xi.y;
+        S.d0 = uj.x - L.F.ui.x;
+        S.d1 = uj.y - L.F.ui.y;
+        S.Vj = a.V[L.j[s]];
+        L.br[s] = S.act ? a.broken[S.b] : (unsigned char)1;
+        L.kinv[s][0] = S.act ? a.kinv[3 * (size_t)S.b] : 0.0;
+        L.kinv[s][1] = S.act ? a.kinv[3 * (size_t)S.b + 1] : 0.0;
+        L.kinv[s][2] = S.act ? a.kinv[3 * (size_t)S.b + 2] : 0.0;
+    }
+}
+
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+// One explicit force evaluation of a family group: bond breaking, K
+// re-assembly for families that lost a bond, then the two family contractions
+// and the force-state outputs.  hook_a runs halfway through contraction 1,
+// hook_b after it (a persistent wave issues its next group's loads there).
+template <typename R, int LPF, int BPL, int N2_IS_10, typename HA, typename HB>
+__device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, double* rec64,
+                                          int lane, const HA& hook_a, const HB& hook_b)
 {
     typedef typename Vec2<R>::T R2;
     constexpr int FS = Tile<LPF, BPL>::FS;
     constexpr int FPW = Tile<LPF, BPL>::FPW;
-    extern __shared__ __attribute__((aligned(16))) double smem[];
-    const int lane = threadIdx.x & 63;
-    const int wib = threadIdx.x >> 6;
-    const int wpb = blockDim.x >> 6;
-    const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
-    const LaneFam F = load_family<LPF>(a, lane, blk * wpb + wib);
-    double* rec64 = smem + wib * Tile<LPF, BPL>::REC * 8;
-
-    BondSlot S[BPL];
+    const LaneFam& F = L.F;
+    BondSlot (&S)[BPL] = L.S;
     bool alive[BPL];
     int nnew = 0;
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
-        S[s] = load_slot(a, F, F.t + s * LPF);
         // ---- bond breaking (FP64, irreversible, both directions agree) ----
-        alive[s] = S[s].act && a.broken[S[s].b] == 0;
+        alive[s] = S[s].act && L.br[s] == 0;
         bool newly = false;
-        if (a.s0 > 0.0 && alive[s]) {
-            const double st = bond_stretch(S[s].x0, S[s].x1, S[s].d0, S[s].d1);
-            if (st > a.s0) {
-                alive[s] = false;
-                newly = true;
-                a.broken[S[s].b] = 1;
-            }
+        if (a.s0 > 0.0 && alive[s] && bond_breaks(S[s].x0, S[s].x1, S[s].d0, S[s].d1, a.brk_c)) {
+            alive[s] = false;
+            newly = true;
+            a.broken[S[s].b] = 1;
         }
         nnew += __popcll(__ballot(newly));
     }
@@ -379,33 +429,40 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     } else {
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
-            kinv[s][0] = kinv[s][1] = kinv[s][2] = 0.0;
-            if (alive[s]) {
-                kinv[s][0] = a.kinv[3 * (size_t)S[s].b];
-                kinv[s][1] = a.kinv[3 * (size_t)S[s].b + 1];
-                kinv[s][2] = a.kinv[3 * (size_t)S[s].b + 2];
-            }
+            kinv[s][0] = L.kinv[s][0];
+            kinv[s][1] = L.kinv[s][1];
+            kinv[s][2] = L.kinv[s][2];
         }
     }
 
     // ---- records for the first contraction ----
+    // FP64 sends V l du and reuses the record's unit vector (P = (V l du) (x) e,
+    // 48 bytes per record instead of 64: the FP64 loop is bound by LDS bytes);
+    // FP32 sends the four products (the FP32 loop is bound by VALU issue).
+    constexpr bool COMPACT = sizeof(R) == 8;
     R* rec = reinterpret_cast<R*>(__builtin_assume_aligned(rec64, 16));
     Geo<R> g[BPL];
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         g[s] = GeoOf<R>::get(a, S[s]);
-        const R x0r = R(S[s].x0), x1r = R(S[s].x1);
-        const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) : R(0);
-        const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) : R(0);
         const int k = S[s].k;
         rec[rec_at<R, FPW>(k, F.slot, 0)] = g[s].ex;
         rec[rec_at<R, FPW>(k, F.slot, 1)] = g[s].ey;
         rec[rec_at<R, FPW>(k, F.slot, 2)] = g[s].pe;
         rec[rec_at<R, FPW>(k, F.slot, 3)] = g[s].qe;
-        rec[rec_at<R, FPW>(k, F.slot, 4)] = Vd0 * x0r;
-        rec[rec_at<R, FPW>(k, F.slot, 5)] = Vd0 * x1r;
-        rec[rec_at<R, FPW>(k, F.slot, 6)] = Vd1 * x0r;
-        rec[rec_at<R, FPW>(k, F.slot, 7)] = Vd1 * x1r;
+        if constexpr (COMPACT) {
+            const double Vl = alive[s] ? S[s].Vj * double(g[s].len) : 0.0;
+            rec[rec_at<R, FPW>(k, F.slot, 4)] = R(Vl * S[s].d0);
+            rec[rec_at<R, FPW>(k, F.slot, 5)] = R(Vl * S[s].d1);
+        } else {
+            const R x0r = R(S[s].x0), x1r = R(S[s].x1);
+            const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) : R(0);
+            const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) : R(0);
+            rec[rec_at<R, FPW>(k, F.slot, 4)] = Vd0 * x0r;
+            rec[rec_at<R, FPW>(k, F.slot, 5)] = Vd0 * x1r;
+            rec[rec_at<R, FPW>(k, F.slot, 6)] = Vd1 * x0r;
+            rec[rec_at<R, FPW>(k, F.slot, 7)] = Vd1 * x1r;
+        }
     }
     // records past a family's end are zero (inactive slots), so the loops may
     // run to the wave's largest family rounded up to 4 without guards
@@ -427,18 +484,33 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
                 const R ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
                 const R pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
                 const R qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
-                const R P0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
-                const R P1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
-                const R P2 = rec[rec_at<R, FPW>(l, F.slot, 6)];
-                const R P3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
+                if constexpr (COMPACT) {
+                    const R a0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
+                    const R a1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
 #pragma unroll
-                for (int s = 0; s < BPL; ++s) {
-                    const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
-                    w[s][l] = om;
-                    D[s][0] += om * P0;
-                    D[s][1] += om * P1;
-                    D[s][2] += om * P2;
-                    D[s][3] += om * P3;
+                    for (int s = 0; s < BPL; ++s) {
+                        const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        w[s][l] = om;
+                        const R t0 = om * a0, t1 = om * a1;
+                        D[s][0] += t0 * ex;
+                        D[s][1] += t0 * ey;
+                        D[s][2] += t1 * ex;
+                        D[s][3] += t1 * ey;
+                    }
+                } else {
+                    const R P0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
+                    const R P1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
+                    const R P2 = rec[rec_at<R, FPW>(l, F.slot, 6)];
+                    const R P3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s) {
+                        const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        w[s][l] = om;
+                        D[s][0] += om * P0;
+                        D[s][1] += om * P1;
+                        D[s][2] += om * P2;
+                        D[s][3] += om * P3;
+                    }
                 }
             }
         } else {
@@ -450,11 +522,12 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
         // keep the scheduler from hoisting every record load of the unrolled
         // loop (which would spill)
         __builtin_amdgcn_sched_barrier(0);
+        if (l0 == FS / 2 - 4) hook_a();
     }
+    hook_b();
 
     // ---- F - I = D K^-1, plane-stress sigma, Q_k = V_k sigma K^-1 ----
-    const double Ei = a.E ? a.E[F.famok ? F.fam : 0] : a.E0;
-    const double pf = Ei / (1.0 - a.nu * a.nu);
+    const double pf = a.pf ? a.pf[F.famok ? F.fam : 0] : a.pf0;
     const R c11 = R(pf), c12 = R(pf * a.nu), c33 = R(pf * (1.0 - a.nu));
     R Q[BPL][4];
 #pragma unroll
@@ -513,6 +586,12 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
 #pragma unroll
     for (int s = 0; s < BPL; ++s) vsum += alive[s] ? S[s].Vj : 0.0;
     const double VH = family_sum<LPF>(vsum);
+    // 1 / V_H: FP64 division, or the hardware reciprocal in FP32
+    R ivh;
+    if constexpr (sizeof(R) == 8)
+        ivh = VH > 0.0 ? 1.0 / VH : 0.0;
+    else
+        ivh = VH > 0.0 ? __builtin_amdgcn_rcpf(float(VH)) : 0.0f;
     R T0[BPL], T1[BPL];
     double fo0 = 0.0, fo1 = 0.0;
 #pragma unroll
@@ -520,7 +599,6 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
         T0[s] = R(0);
         T1[s] = R(0);
         if (alive[s]) {
-            const R ivh = R(1.0 / VH);
             const R x0r = R(S[s].x0), x1r = R(S[s].x1);
             T0[s] = (Sm[s][0] * x0r + Sm[s][1] * x1r) * ivh;
             T1[s] = (Sm[s][2] * x0r + Sm[s][3] * x1r) * ivh;
@@ -541,6 +619,63 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
             t.y = T1[s] * R(F.Vi);
             reinterpret_cast<R2*>(a.W)[a.rev[S[s].b]] = t;
         }
+    }
+}
+
+// Non-persistent launch: one wave per family group.
+template <typename R, int LPF, int BPL, int N2_IS_10>
+__global__ void __launch_bounds__(512) k_bond(BondArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
+    FamLoad<BPL> L;
+    stage_rows<LPF, BPL>(a, L, lane, blk * wpb + wib);
+    stage_cols<LPF, BPL>(a, L);
+    stage_bonds<LPF, BPL>(a, L);
+    bond_body<R, LPF, BPL, N2_IS_10>(a, L, smem + wib * Tile<LPF, BPL>::REC * 8, lane, NoHook(),
+                                     NoHook());
+}
+
+// Persistent launch: each block owns a contiguous range of family groups (per
+// XCD after the remap); each of its waves walks the range with stride wpb and
+// issues the next group's three load stages inside the current group's
+// contraction, so the dependent gathers overlap compute.  Every wave leaves
+// the loop when its range is done (no grid-wide wait).
+template <typename R, int LPF, int BPL, int N2_IS_10>
+__global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
+{
+    constexpr int FPW = Tile<LPF, BPL>::FPW;
+    extern __shared__ __attribute__((aligned(16))) double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
+    const int ngroups = (a.n + FPW - 1) / FPW;
+    const int per = ngroups / (int)gridDim.x, rem = ngroups % (int)gridDim.x;
+    const int g0 = blk * per + (blk < rem ? blk : rem);
+    const int g1 = g0 + per + (blk < rem ? 1 : 0);
+    double* rec64 = smem + wib * Tile<LPF, BPL>::REC * 8;
+    int g = g0 + wib;
+    if (g >= g1) return;
+    FamLoad<BPL> cur, nxt;
+    stage_rows<LPF, BPL>(a, cur, lane, g);
+    stage_cols<LPF, BPL>(a, cur);
+    stage_bonds<LPF, BPL>(a, cur);
+    for (;;) {
+        const int gn = g + wpb;
+        const bool more = gn < g1;
+        if (more) stage_rows<LPF, BPL>(a, nxt, lane, gn);
+        bond_body<R, LPF, BPL, N2_IS_10>(
+            a, cur, rec64, lane,
+            [&]() { if (more) stage_cols<LPF, BPL>(a, nxt); },
+            [&]() { if (more) stage_bonds<LPF, BPL>(a, nxt); });
+        if (!more) break;
+        cur = nxt;
+        g = gn;
+        wave_sync();
     }
 }
 
@@ -636,25 +771,38 @@ __global__ void k_fill_u32(unsigned int* p, unsigned int v, int count)
     if (t < count) p[t] = v;
 }
 
+// Plain streaming copy of n4 32-byte elements (grid-stride), the reference
+// kernel of known traffic for bandwidth and HBM-counter calibration.
+__global__ void __launch_bounds__(256) k_copy_probe(const double4* __restrict__ src,
+                                                    double4* __restrict__ dst, long n4)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
 template <typename R, int LPF, int BPL, int N2>
-static hipError_t launch_bond_t(const BondArgs& a, int wpb, hipStream_t s)
+static hipError_t launch_bond_t(const BondArgs& a, int wpb, int pgrid, hipStream_t s)
 {
     const int grid = ceil_div(a.n, (long)wpb * Tile<LPF, BPL>::FPW);
     if (grid == 0) return hipSuccess;
     const size_t lds = (size_t)wpb * Tile<LPF, BPL>::REC * 8 * sizeof(double);
-    hipLaunchKernelGGL((k_bond<R, LPF, BPL, N2>), dim3(grid), dim3(64 * wpb), lds, s, a);
+    if (pgrid > 0 && pgrid < grid)
+        hipLaunchKernelGGL((k_bond_p<R, LPF, BPL, N2>), dim3(pgrid), dim3(64 * wpb), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_bond<R, LPF, BPL, N2>), dim3(grid), dim3(64 * wpb), lds, s, a);
     return hipGetLastError();
 }
 
 template <typename R, int LPF, int BPL>
-static hipError_t launch_bond_n2(const BondArgs& a, int n2_is_10, int wpb, hipStream_t s)
+static hipError_t launch_bond_n2(const BondArgs& a, int n2_is_10, int wpb, int pgrid, hipStream_t s)
 {
-    return n2_is_10 ? launch_bond_t<R, LPF, BPL, 1>(a, wpb, s) : launch_bond_t<R, LPF, BPL, 0>(a, wpb, s);
+    return n2_is_10 ? launch_bond_t<R, LPF, BPL, 1>(a, wpb, pgrid, s)
+                    : launch_bond_t<R, LPF, BPL, 0>(a, wpb, pgrid, s);
 }
 
 bool tile_supported(int mixed, int lpf, int bpl)
@@ -665,17 +813,17 @@ bool tile_supported(int mixed, int lpf, int bpl)
 }
 
 hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int n2_is_10, int wpb,
-                       hipStream_t s)
+                       int pgrid, hipStream_t s)
 {
     if (mixed) {
-        if (bpl == 1 && lpf == 32) return launch_bond_n2<float, 32, 1>(a, n2_is_10, wpb, s);
-        if (bpl == 1 && lpf == 64) return launch_bond_n2<float, 64, 1>(a, n2_is_10, wpb, s);
-        if (bpl == 2 && lpf == 16) return launch_bond_n2<float, 16, 2>(a, n2_is_10, wpb, s);
-        if (bpl == 2 && lpf == 32) return launch_bond_n2<float, 32, 2>(a, n2_is_10, wpb, s);
+        if (bpl == 1 && lpf == 32) return launch_bond_n2<float, 32, 1>(a, n2_is_10, wpb, pgrid, s);
+        if (bpl == 1 && lpf == 64) return launch_bond_n2<float, 64, 1>(a, n2_is_10, wpb, pgrid, s);
+        if (bpl == 2 && lpf == 16) return launch_bond_n2<float, 16, 2>(a, n2_is_10, wpb, pgrid, s);
+        if (bpl == 2 && lpf == 32) return launch_bond_n2<float, 32, 2>(a, n2_is_10, wpb, pgrid, s);
     } else {
-        if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1>(a, n2_is_10, wpb, s);
-        if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1>(a, n2_is_10, wpb, s);
-        if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2>(a, n2_is_10, wpb, s);
+        if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1>(a, n2_is_10, wpb, pgrid, s);
+        if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1>(a, n2_is_10, wpb, pgrid, s);
+        if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2>(a, n2_is_10, wpb, pgrid, s);
     }
     return hipErrorInvalidValue;
 }
@@ -737,6 +885,15 @@ hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(ceil_div(count, 256)), dim3(256), 0, s, p, v, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_probe(const double4* src, double4* dst, long n4, hipStream_t s)
+{
+    if (n4 <= 0) return hipSuccess;
+    const long want = (n4 + 255) / 256;
+    const int grid = (int)(want < 256L * 64 ? want : 256L * 64);
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, s, src, dst, n4);
     return hipGetLastError();
 }
 

@@ -34,6 +34,7 @@ OPT_PROFILE = 2
 OPT_WAVES_PER_WG = 3
 OPT_XCD_REMAP = 4
 OPT_TILE = 5
+OPT_PERSISTENT = 6
 
 # every entry point include/pd_hip.h declares
 EXPORTS = (
@@ -43,7 +44,7 @@ EXPORTS = (
     "pd_apply_velocity_bc", "pd_apply_fixed_bc", "pd_update_positions",
     "pd_compute_velocity", "pd_set_boundary", "pd_step", "pd_autotune",
     "pd_get_broken", "pd_set_broken", "pd_get_bond_forces", "pd_get_stats",
-    "pd_reset_profile", "pd_build_families",
+    "pd_reset_profile", "pd_build_families", "pd_notch_mask", "pd_probe_copy",
 )
 
 
@@ -73,7 +74,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("n", C.c_int), ("nb", C.c_long), ("n_broken", C.c_long), ("n_singular", C.c_long),
         ("max_family", C.c_int), ("lanes_per_family", C.c_int), ("bonds_per_lane", C.c_int),
-        ("steps", C.c_long),
+        ("persistent", C.c_int), ("waves_per_wg", C.c_int), ("steps", C.c_long),
         ("phase_ms", C.c_double * 4), ("phase_calls", C.c_long * 4),
     ]
 
@@ -119,6 +120,8 @@ def lib():
         "pd_get_stats": (I, [ctx, C.POINTER(Stats)]),
         "pd_reset_profile": (None, [ctx]),
         "pd_build_families": (C.c_long, [I, P, C.c_double, C.c_double, P, P, C.c_long]),
+        "pd_notch_mask": (C.c_long, [I, P, P, P, I, P, P]),
+        "pd_probe_copy": (I, [C.c_long, I, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -157,6 +160,16 @@ def build_families(X, dx, horizon):
     if nb2 != nb:
         raise PDError(PD_ERR_INVALID, "pd_build_families failed")
     return row, col[:nb]
+
+
+def probe_copy(nbytes=1 << 30, iters=20):
+    """Achievable HBM rate (GB/s) of a plain copy kernel reading and writing
+    `nbytes`; also the kernel of known traffic for counter calibration."""
+    g = C.c_double()
+    rc = lib().pd_probe_copy(int(nbytes), int(iters), C.byref(g))
+    if rc != PD_OK:
+        raise PDError(rc, lib().pd_last_error(None).decode())
+    return g.value
 
 
 class Context:

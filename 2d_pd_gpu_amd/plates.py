@@ -116,18 +116,8 @@ def notch_mask(X, row, col, segments):
     return mask[: len(col)]
 
 
-def _lib_setup():
-    L = pdhip.lib()
-    if not hasattr(L, "_notch_sig"):
-        P = pdhip.C.c_void_p
-        L.pd_notch_mask.restype = pdhip.C.c_long
-        L.pd_notch_mask.argtypes = [pdhip.C.c_int, P, P, P, pdhip.C.c_int, P, P]
-        L._notch_sig = True
-
-
 def single_plate(name, nx, ny, v, notch_len_cells=None, notch_kind="center", horizon=3.015,
                  seed=0, E_sigma=0.0, porosity=0.0, notches=None, dx=DX):
-    _lib_setup()
     X, iy = plate_points(nx, ny, dx=dx, seed=seed, porosity=porosity)
     n = X.size // 2
     V = np.full(n, dx * dx * H)
@@ -184,7 +174,6 @@ def plate_4096(seed=0, n=4096):
 def plates_batched(count=96, seed=0, lo=64, hi=512):
     """Many independent plates as one disjoint-union mesh (plates side by side
     with a gap wider than the horizon, so no bond crosses plates)."""
-    _lib_setup()
     rng = _rng(seed, 11)
     sizes = rng.integers(lo, hi + 1, size=(count, 2))
     nl = rng.uniform(0.1, 0.4, size=count)

@@ -129,6 +129,10 @@ def main():
     ap.add_argument("--tune", type=int, default=3, help="autotuner steps per variant (0: off)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
+                    help="run one precision only (profiling runs); prints no JSON line")
+    ap.add_argument("--probe-copy", type=int, default=1 << 30,
+                    help="bytes of the plain copy probe (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the step shards by space across GPUs "
@@ -141,9 +145,16 @@ def main():
     print(f"# workload {w.name}: n={w.n} nb={w.nb} pre-broken={int(w.broken.sum())} "
           f"setup {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
 
+    copy_gbs = None
+    if args.probe_copy:
+        copy_gbs = pkg.pdhip.probe_copy(args.probe_copy, 20)
+        print(f"# copy probe {args.probe_copy} B: {copy_gbs:.1f} GB/s", file=sys.stderr, flush=True)
+
     res = {}
     for name, prec in (("fp64", pkg.FP64), ("mixed", pkg.MIXED)):
         if name == "mixed" and args.no_mixed:
+            continue
+        if args.only and name != args.only:
             continue
         t_graph, t_prof, st, state = run(pkg, w, prec, args.steps, args.warmup, args.tune)
         nb_alive = w.nb - st.n_broken
@@ -159,6 +170,8 @@ def main():
               f"bond {bond_ms:.4f} ms gather {gather_ms:.4f} ms positions {pos_ms:.4f} ms "
               f"broken {st.n_broken}", file=sys.stderr, flush=True)
 
+    if args.only:
+        return
     r = res["fp64"]
     ms_per_step = 1e3 * r["t"] / args.steps
     value = r["nb_alive"] * args.steps / r["t"]
@@ -180,6 +193,7 @@ def main():
                    "family_max": int(np.diff(w.row).max()), "steps": args.steps},
         "particle_steps_per_s": w.n * args.steps / r["t"],
         "step_hbm_frac": r["sb"] / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "copy_probe_gbs": copy_gbs,
         "kernel_ms": {"bond": r["bond_ms"], "gather": r["gather_ms"], "positions": r["pos_ms"]},
         "tile": {"lanes_per_family": r["tile"][0], "bonds_per_lane": r["tile"][1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

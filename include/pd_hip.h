@@ -63,6 +63,9 @@ extern "C" {
 #define PD_OPT_TILE          5  /* family tile: 100*lanes_per_family + bonds_per_lane:
                                    3201, 6401, 1602 (and 3202 in mixed precision);
                                    0 = 3201 or 6401 by the largest family            */
+#define PD_OPT_PERSISTENT    6  /* bond kernel as a persistent grid of value workgroups
+                                   per CU that prefetch their next family group
+                                   (0 = one wave per family group, the default)      */
 
 typedef struct pd_params {
     int    ndim;       /* must be 2 */
@@ -86,6 +89,8 @@ typedef struct pd_stats {
     int    max_family;     /* largest family */
     int    lanes_per_family;
     int    bonds_per_lane;
+    int    persistent;     /* PD_OPT_PERSISTENT in effect */
+    int    waves_per_wg;   /* PD_OPT_WAVES_PER_WG in effect */
     long   steps;          /* steps run by pd_step since creation */
     double phase_ms[4];    /* profiled time: positions, bonds, gather, other */
     long   phase_calls[4];
@@ -163,6 +168,13 @@ int pd_set_broken(pd_context* ctx, const unsigned char* mask);
 int pd_get_bond_forces(pd_context* ctx, double* T);
 int pd_get_stats(pd_context* ctx, pd_stats* out);
 void pd_reset_profile(pd_context* ctx);
+
+/* ---- diagnostics (no counterpart in the reference) ---------------------------
+ * Times a plain device-to-device copy kernel that reads and writes `bytes`
+ * bytes, `iters` times on the current device; *gbps = 2*bytes*iters / time.
+ * Gives the achievable HBM rate next to the roofline and a kernel of known
+ * traffic to scale HBM counter readings (FETCH_SIZE / WRITE_SIZE) against. */
+int pd_probe_copy(long bytes, int iters, double* gbps);
 
 /* ---- setup helpers (host side; no counterpart in the hot path) -------------- */
 /* Families with the reference cutoff |xi| < 1.01*dx*horizon

@@ -59,18 +59,38 @@ double or_bond_stretch(double xi0, double xi1, double du0, double du1)
     return (cur - len) / len;
 }
 
+/* s > s0  <=>  |xi + du|^2 > (1 + s0)^2 |xi|^2 (both sides positive); evaluated
+ * with every operation rounded on its own and c = (1 + s0)^2 formed once, so
+ * both directions of a bond agree bit for bit. */
+int or_bond_breaks(double xi0, double xi1, double du0, double du1, double c)
+{
+    volatile double e0 = xi0 + du0;
+    volatile double e1 = xi1 + du1;
+    volatile double q0 = e0 * e0;
+    volatile double q1 = e1 * e1;
+    volatile double p0 = xi0 * xi0;
+    volatile double p1 = xi1 * xi1;
+    volatile double cur2 = q0 + q1;
+    volatile double len2 = p0 + p1;
+    volatile double lim = c * len2;
+    return cur2 > lim;
+}
+
 long or_break_bonds(const or_params* p, int n, const double* X, const int* row,
                     const int* col, const double* u, unsigned char* broken)
 {
     long nnew = 0;
     if (!(p->s0 > 0.0)) return 0;
+    const double c = (1.0 + p->s0) * (1.0 + p->s0);
     for (int i = 0; i < n; ++i) {
         for (int b = row[i]; b < row[i + 1]; ++b) {
             if (broken[b]) continue;
             int j = col[b];
-            double s = or_bond_stretch(X[2 * j] - X[2 * i], X[2 * j + 1] - X[2 * i + 1],
-                                       u[2 * j] - u[2 * i], u[2 * j + 1] - u[2 * i + 1]);
-            if (s > p->s0) { broken[b] = 1; ++nnew; }
+            if (or_bond_breaks(X[2 * j] - X[2 * i], X[2 * j + 1] - X[2 * i + 1],
+                               u[2 * j] - u[2 * i], u[2 * j + 1] - u[2 * i + 1], c)) {
+                broken[b] = 1;
+                ++nnew;
+            }
         }
     }
     return nnew;

@@ -77,6 +77,11 @@ long or_forces_fp64(const or_params* p, int n, const double* X, const double* V,
  * no fused operations, identical for both directions of a bond. */
 double or_bond_stretch(double xi0, double xi1, double du0, double du1);
 
+/* Breaking test of the project rule s > s0, evaluated as
+ * |xi + du|^2 > c |xi|^2 with c = (1 + s0)^2, every operation rounded on its
+ * own (identical bits for (i->j) and (j->i)). */
+int or_bond_breaks(double xi0, double xi1, double du0, double du1, double c);
+
 /* Mark bonds whose stretch exceeds s0 as broken (irreversible).  Returns the
  * number of newly broken directed bonds. */
 long or_break_bonds(const or_params* p, int n, const double* X, const int* row,

@@ -48,6 +48,8 @@ def lib():
         L.or_forces_fp64.argtypes = [C.POINTER(OrParams), C.c_int] + [P] * 8 + [C.c_int, P, P, P, C.c_int]
         L.or_bond_stretch.restype = C.c_double
         L.or_bond_stretch.argtypes = [C.c_double] * 4
+        L.or_bond_breaks.restype = C.c_int
+        L.or_bond_breaks.argtypes = [C.c_double] * 5
         L.or_break_bonds.restype = C.c_long
         L.or_break_bonds.argtypes = [C.POINTER(OrParams), C.c_int, P, P, P, P, P]
         L.or_step_fp64.restype = C.c_long
@@ -107,6 +109,11 @@ def forces(p, X, V, row, col, u, broken=None, E_arr=None, rev=None, bugcompat=Fa
 
 def bond_stretch(xi, du):
     return lib().or_bond_stretch(float(xi[0]), float(xi[1]), float(du[0]), float(du[1]))
+
+
+def bond_breaks(xi, du, s0):
+    c = (1.0 + s0) * (1.0 + s0)
+    return bool(lib().or_bond_breaks(float(xi[0]), float(xi[1]), float(du[0]), float(du[1]), c))
 
 
 def break_bonds(p, X, row, col, u, broken):

@@ -266,12 +266,14 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     mask = small_plate.code >= 0
     for tile in (3201, 6401, 1602):
         tile_ref = None
-        for wpb in (1, 2, 8):
-            for xcd in (0, 1):
+        for wpb, xcd, persist in ((1, 0, 0), (2, 1, 0), (8, 0, 0), (8, 1, 0), (4, 1, 1), (2, 0, 3),
+                                  (8, 1, 2)):
+            if True:
                 ctx = small_plate.context(s0=0.005)
                 ctx.set_option(pdh.OPT_WAVES_PER_WG, wpb)
                 ctx.set_option(pdh.OPT_XCD_REMAP, xcd)
                 ctx.set_option(pdh.OPT_TILE, tile)
+                ctx.set_option(pdh.OPT_PERSISTENT, persist)
                 ctx.step(40)
                 v = ctx.get_state()["v"]
                 np.testing.assert_array_equal(ctx.get_broken(), bref)
@@ -309,6 +311,29 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     assert nonbc < 1e-13
     np.testing.assert_array_equal(ctx.get_broken(), bref)
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_persistent_bond_kernel_bitwise(pdpkg, precision):
+    """The persistent bond kernel (each wave walks many family groups and
+    prefetches the next one) gives the same bits as one wave per group."""
+    say(f"persistent bond kernel, 200 x 200 plate, precision {precision}")
+    pdh = pdpkg.pdhip
+    w = pdpkg.plates.single_plate("p200", 200, 200, 40.0, notch_len_cells=40, seed=8)
+    out = []
+    for persist, wpb in ((0, 4), (1, 4), (2, 2), (1, 8)):
+        ctx = w.context(precision=precision, s0=0.003)
+        ctx.set_option(pdh.OPT_PERSISTENT, persist)
+        ctx.set_option(pdh.OPT_WAVES_PER_WG, wpb)
+        ctx.step(30)
+        st = ctx.get_state()
+        out.append((st["v"], st["f"], ctx.get_broken()))
+        ctx.close()
+    assert out[0][2].sum() > w.broken.sum()  # bonds broke during the run
+    for v, f, b in out[1:]:
+        np.testing.assert_array_equal(v, out[0][0])
+        np.testing.assert_array_equal(f, out[0][1])
+        np.testing.assert_array_equal(b, out[0][2])
 
 
 def test_per_particle_youngs_modulus(pdpkg):

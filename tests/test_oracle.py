@@ -342,10 +342,15 @@ def test_broken_bonds_equal_removed_bonds(jittered):
 
 def test_stretch_symmetric_and_breaking_rule():
     rng = np.random.default_rng(7)
-    for _ in range(200):
+    for _ in range(2000):
         xi = rng.normal(size=2)
         du = rng.normal(size=2) * 0.03
         assert O.bond_stretch(xi, du) == O.bond_stretch(-xi, -du)
+        # the squared-length test decides exactly like s > s0 away from ties
+        s = O.bond_stretch(xi, du)
+        assert O.bond_breaks(xi, du, 0.02) == O.bond_breaks(-xi, -du, 0.02)
+        if abs(s - 0.02) > 1e-12:
+            assert O.bond_breaks(xi, du, 0.02) == (s > 0.02)
     X = grid(3, 1)
     n = 3
     row, col = families(X, 1.0, 3.015)
