@@ -1,0 +1,856 @@
+// pd_bond.h -- device code of the bond kernel (family contractions, breaking,
+// force-state outputs) shared by the per-precision instantiation units.
+//
+// Math (EricYJA/2D_PD_GPU, src/main_seq.cpp):
+//   w_i(j,k) = exp(-n1 |l_ij - l_ik| / (delta dx)) (1/2 + 1/2 cos theta_jk)^n2      :103-118
+//   K_ik     = sum_l w_i(k,l) V_l xi_l (x) xi_l                                      :135-140
+//   N_ik     = K_ik + D_ik,  D_ik = sum_l w_i(k,l) V_l du_l (x) xi_l  (u = y - X)    :138
+//   F_ik - I = D_ik K_ik^-1,  eps = sym(F - I),  sigma = C eps (plane stress)        :224-243
+//   T_ij     = (1/V_H) [sum_k w_i(j,k) V_k sigma_ik K_ik^-1] xi_ij                   :298-371
+//   f_i      = sum_j (T_ij - T_ji) V_j,  v += 1/2 (a + f/rho) dt                     :411-466
+// The reference evaluates K_ik, N_ik inside the j loop (O(M^3) per particle);
+// they do not depend on j, and w is symmetric, so each family is evaluated once
+// here (O(M^2)).  Displacement form keeps F - I free of cancellation.
+//
+// Layout: a family of up to FS = LPF * BPL bonds is handled by LPF lanes, each
+// owning BPL bonds; a wave holds 64 / LPF families.  Each lane keeps its weight
+// rows w(k, .) in registers; the per-bond operands that every lane of a family
+// needs (unit bond vector, radial factors, V du (x) xi, later V sigma K^-1) go
+// through per-wave LDS records that the l-loop reads as broadcasts.  No float
+// atomics: each family writes its own-bond sum and, into the reverse-bond slot
+// of each neighbour's row, the term that neighbour must subtract; the gather
+// kernel then reads one contiguous row per particle.
+#pragma once
+
+#include "pd_internal.h"
+
+namespace pd {
+
+// Order LDS traffic between the lanes of one wavefront.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Consecutive blocks land on different XCDs (round-robin dispatch); remap so
+// that each XCD works through one contiguous range of families and keeps their
+// neighbours in its own L2.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks, int nxcd)
+{
+    if (nxcd <= 1) return b;
+    int x = b % nxcd, idx = b / nxcd;
+    int per = nblocks / nxcd, rem = nblocks % nxcd;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+
+// Breaking test s = (|xi + du| - |xi|) / |xi| > s0, evaluated as
+// |xi + du|^2 > c |xi|^2 with c = (1 + s0)^2 and every operation rounded on its
+// own, so (i->j) and (j->i) give the same bits; matches the oracle's
+// or_bond_breaks.
+__device__ __forceinline__ bool bond_breaks(double x0, double x1, double d0, double d1, double c)
+{
+#pragma clang fp contract(off)
+    const double e0 = x0 + d0;
+    const double e1 = x1 + d1;
+    const double cur2 = e0 * e0 + e1 * e1;
+    const double len2 = x0 * x0 + x1 * x1;
+    return cur2 > c * len2;
+}
+
+__device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double rmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double rpow(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float rpow(float a, float b) { return powf(a, b); }
+
+// w(k,l) = exp(-A |l_k - l_l|) (1/2 + 1/2 cos theta_kl)^n2 from the k-side half
+// unit vector (hx, hy) = e_k / 2, the l-side unit vector, and the radial
+// factors p = exp(A l), q = exp(-A l): exp(-A |l_k - l_l|) = min(q_k p_l, p_k q_l).
+// 1/2 + 1/2 cos lies in [0, 1] up to rounding; for n2 = 10 the power is four
+// squarings and needs no clamp (a value a few ulp outside moves w by a few ulp),
+// the general power clamps at 0 (src/main_seq.cpp:115 clamps cos to [-1, 1]).
+template <typename R, int N2_IS_10>
+__device__ __forceinline__ R weight(R hxk, R hyk, R pk, R qk, R exl, R eyl, R pl, R ql, R n2)
+{
+    const R h = hxk * exl + (hyk * eyl + R(0.5));
+    R hp;
+    if constexpr (N2_IS_10) {
+        const R h2 = h * h;
+        const R h4 = h2 * h2;
+        const R h8 = h4 * h4;
+        hp = h8 * h2;
+    } else {
+        hp = rpow(rmax(h, R(0)), n2);
+    }
+    return rmin(qk * pl, pk * ql) * hp;
+}
+
+// Sum over the lanes of one family (LPF lanes, aligned).
+template <int LPF>
+__device__ __forceinline__ double family_sum(double x)
+{
+#pragma unroll
+    for (int off = LPF / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+template <typename R> struct Vec2;
+template <> struct Vec2<double> { typedef double2 T; };
+template <> struct Vec2<float> { typedef float2 T; };
+
+// Launch tile: LPF lanes per family, BPL bonds per lane (lane t owns bonds
+// t, t + LPF, ...), so a family has FS = LPF * BPL slots and a wave holds
+// FPW = 64 / LPF families.  More bonds per lane cut the LDS broadcast traffic
+// per weight evaluation (each record read feeds BPL rows) at the cost of
+// registers for BPL weight rows.
+template <int LPF, int BPL> struct Tile {
+    static constexpr int FS = LPF * BPL;
+    static constexpr int FPW = 64 / LPF;
+    static constexpr int REC = 64 * BPL;  // LDS records (8 values each) per wave
+};
+
+// LDS record layout.  Value e (0..7) of record k of family `slot` sits in
+// 16-byte chunk e / VPC of that record; chunks are interleaved across the
+// wave's FPW families, so one broadcast read (every lane of a family asks for
+// the same chunk) touches FPW consecutive 16-byte words in distinct banks.
+template <typename T, int FPW>
+__device__ __forceinline__ int rec_at(int k, int slot, int e)
+{
+    constexpr int VPC = 16 / (int)sizeof(T);
+    constexpr int NCH = 8 / VPC;
+    return ((k * NCH + e / VPC) * FPW + slot) * VPC + e % VPC;
+}
+
+// The family a lane works on.  Loads are issued without branches (indices
+// clamped to valid entries, values masked where they are used), so that a
+// wave can keep a later group's loads in flight across its current work: the
+// vector-memory counter is waited on in issue order, and a load skipped on one
+// path makes every wait after the merge a full drain.
+struct LaneFam {
+    int fam, t, slot, r0, r1, beg, M;
+    bool famok;
+    double2 Xi, ui;
+    double Vi, pf;
+};
+
+template <int LPF>
+__device__ __forceinline__ LaneFam load_family(const BondArgs& a, int lane, int wave_global)
+{
+    LaneFam F;
+    F.slot = lane / LPF;
+    F.t = lane % LPF;
+    F.fam = wave_global * (64 / LPF) + F.slot;
+    F.famok = F.fam < a.n;
+    const int fi = F.famok ? F.fam : 0;
+    F.r0 = a.row[fi];
+    F.r1 = a.row[fi + 1];
+    F.Xi = a.X[fi];
+    F.ui = a.u[fi];
+    F.Vi = a.V[fi];
+    F.pf = a.pf[fi & a.pf_mask];
+    return F;
+}
+
+// Family extent from the row pointers (first use of the row loads).
+__device__ __forceinline__ void family_extent(LaneFam& F)
+{
+    F.beg = F.r0;
+    F.M = F.famok ? F.r1 - F.r0 : 0;
+}
+
+// One bond slot of a lane.
+struct BondSlot {
+    int b, k;
+    bool act;
+    double x0, x1, d0, d1, Vj;
+};
+
+__device__ __forceinline__ BondSlot load_slot(const BondArgs& a, const LaneFam& F, int k)
+{
+    BondSlot S;
+    S.k = k;
+    S.act = F.famok && k < F.M;
+    S.b = F.beg + k;
+    const int j = S.act ? a.col[S.b] : (F.famok ? F.fam : 0);
+    const double2 Xj = a.X[j], uj = a.u[j];
+    S.x0 = Xj.x - F.Xi.x;
+    S.x1 = Xj.y - F.Xi.y;
+    S.d0 = uj.x - F.ui.x;
+    S.d1 = uj.y - F.ui.y;
+    S.Vj = a.V[j];
+    return S;
+}
+
+// Bond geometry for the weight: unit vector e, half vector e/2 and the radial
+// factors exp(+-A l).  Inactive slots get all zeros, so their weight is 0.
+template <typename R> struct Geo {
+    R ex, ey, hx, hy, pe, qe, len;
+};
+
+__device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondSlot& S)
+{
+    Geo<double> g;
+    const double len = sqrt(S.x0 * S.x0 + S.x1 * S.x1);
+    const double il = S.act ? 1.0 / len : 0.0;
+    g.len = len;
+    g.ex = S.x0 * il;
+    g.ey = S.x1 * il;
+    g.hx = 0.5 * g.ex;
+    g.hy = 0.5 * g.ey;
+    g.pe = S.act ? exp(a.A * len) : 0.0;
+    g.qe = S.act ? exp(-a.A * len) : 0.0;
+    return g;
+}
+
+// FP32 geometry for the mixed-precision contractions (hardware exp / rcp).
+__device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSlot& S)
+{
+    Geo<float> g;
+    const float x0 = float(S.x0), x1 = float(S.x1);
+    const float len = sqrtf(x0 * x0 + x1 * x1);
+    const float il = S.act ? __builtin_amdgcn_rcpf(len) : 0.0f;
+    g.len = len;
+    g.ex = x0 * il;
+    g.ey = x1 * il;
+    g.hx = 0.5f * g.ex;
+    g.hy = 0.5f * g.ey;
+    const float Al = float(a.A) * len;
+    g.pe = S.act ? __expf(Al) : 0.0f;
+    g.qe = S.act ? __expf(-Al) : 0.0f;
+    return g;
+}
+
+template <typename R> struct GeoOf;
+template <> struct GeoOf<double> {
+    static __device__ __forceinline__ Geo<double> get(const BondArgs& a, const BondSlot& S) { return geometry64(a, S); }
+};
+template <> struct GeoOf<float> {
+    static __device__ __forceinline__ Geo<float> get(const BondArgs& a, const BondSlot& S) { return geometry32(a, S); }
+};
+
+// FP64 assembly of K_ik over the alive bonds of each family in the wave, and
+// its inverse under the scale-relative singularity rule.  rec: this wave's LDS
+// records (Tile::REC x 8 doubles).  Every lane of the wave must call it.
+template <int LPF, int BPL, int N2_IS_10>
+__device__ __forceinline__ void assemble_kinv(double* rec, const LaneFam& F, const BondSlot* S,
+                                              const bool* alive, const Geo<double>* g, double n2,
+                                              double ktol, double (*kinv)[3], bool* singular)
+{
+    constexpr int FS = Tile<LPF, BPL>::FS;
+    constexpr int FPW = Tile<LPF, BPL>::FPW;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        const int k = S[s].k;
+        const double V = alive[s] ? S[s].Vj : 0.0;
+        rec[rec_at<double, FPW>(k, F.slot, 0)] = g[s].ex;
+        rec[rec_at<double, FPW>(k, F.slot, 1)] = g[s].ey;
+        rec[rec_at<double, FPW>(k, F.slot, 2)] = g[s].pe;
+        rec[rec_at<double, FPW>(k, F.slot, 3)] = g[s].qe;
+        rec[rec_at<double, FPW>(k, F.slot, 4)] = V * S[s].x0 * S[s].x0;
+        rec[rec_at<double, FPW>(k, F.slot, 5)] = V * S[s].x0 * S[s].x1;
+        rec[rec_at<double, FPW>(k, F.slot, 6)] = V * S[s].x1 * S[s].x1;
+    }
+    wave_sync();
+    double K[BPL][3];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) K[s][0] = K[s][1] = K[s][2] = 0.0;
+#pragma unroll 2
+    for (int l = 0; l < FS; ++l) {
+        const double ex = rec[rec_at<double, FPW>(l, F.slot, 0)];
+        const double ey = rec[rec_at<double, FPW>(l, F.slot, 1)];
+        const double pe = rec[rec_at<double, FPW>(l, F.slot, 2)];
+        const double qe = rec[rec_at<double, FPW>(l, F.slot, 3)];
+        const double kxx = rec[rec_at<double, FPW>(l, F.slot, 4)];
+        const double kxy = rec[rec_at<double, FPW>(l, F.slot, 5)];
+        const double kyy = rec[rec_at<double, FPW>(l, F.slot, 6)];
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) {
+            const double w = weight<double, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2);
+            K[s][0] += w * kxx;
+            K[s][1] += w * kxy;
+            K[s][2] += w * kyy;
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        const double det = K[s][0] * K[s][2] - K[s][1] * K[s][1];
+        const double tr = K[s][0] + K[s][2];
+        singular[s] = false;
+        if (alive[s] && det > ktol * tr * tr) {
+            kinv[s][0] = K[s][2] / det;
+            kinv[s][1] = -K[s][1] / det;
+            kinv[s][2] = K[s][0] / det;
+        } else {
+            kinv[s][0] = kinv[s][1] = kinv[s][2] = 0.0;
+            singular[s] = alive[s];
+        }
+    }
+}
+
+template <int LPF>
+__device__ __forceinline__ int wave_max_family(int m)
+{
+#pragma unroll
+    for (int off = LPF; off < 64; off <<= 1) m = max(m, __shfl_xor(m, off, 64));
+    return __builtin_amdgcn_readfirstlane(m);
+}
+
+// K_ik^-1 for every bond of every family (context creation, pd_set_broken).
+template <int LPF, int BPL, int N2_IS_10>
+__global__ void __launch_bounds__(512) k_shape(BondArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
+    LaneFam F = load_family<LPF>(a, lane, blk * wpb + wib);
+    family_extent(F);
+    BondSlot S[BPL];
+    bool alive[BPL];
+    Geo<double> g[BPL];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        S[s] = load_slot(a, F, F.t + s * LPF);
+        alive[s] = S[s].act && a.broken[S[s].b] == 0;
+        g[s] = geometry64(a, S[s]);
+    }
+    double kinv[BPL][3];
+    bool sing[BPL];
+    assemble_kinv<LPF, BPL, N2_IS_10>(smem + wib * Tile<LPF, BPL>::REC * 8, F, S, alive, g, a.n2,
+                                      a.ktol, kinv, sing);
+    int nsing = 0;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        if (S[s].act) {
+            a.kinv[3 * (size_t)S[s].b] = kinv[s][0];
+            a.kinv[3 * (size_t)S[s].b + 1] = kinv[s][1];
+            a.kinv[3 * (size_t)S[s].b + 2] = kinv[s][2];
+        }
+        nsing += __popcll(__ballot(sing[s]));
+    }
+    if (lane == 0 && nsing) atomicAdd(a.counters + 1, (unsigned int)nsing);
+}
+
+// Global inputs of one family group for one lane, gathered in three dependent
+// stages (row pointers -> neighbour ids -> neighbour state, broken flags and
+// K^-1), so that a persistent wave can issue each stage for its next group
+// while it computes the current one.
+template <int BPL> struct FamLoad {
+    LaneFam F;
+    BondSlot S[BPL];
+    int jr[BPL];              // neighbour id as loaded (valid where S.act)
+    int rv[BPL];              // reverse bond
+    unsigned char br[BPL];
+    double kinv[BPL][3];
+};
+
+template <int LPF, int BPL>
+__device__ __forceinline__ void stage_rows(const BondArgs& a, FamLoad<BPL>& L, int lane, int group)
+{
+    L.F = load_family<LPF>(a, lane, group);
+}
+
+template <int LPF, int BPL>
+__device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<BPL>& L)
+{
+    family_extent(L.F);
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        BondSlot& S = L.S[s];
+        S.k = L.F.t + s * LPF;
+        S.act = L.F.famok && S.k < L.F.M;
+        S.b = L.F.beg + S.k;
+        L.jr[s] = a.col[S.act ? S.b : 0];  // the bond arrays hold at least one entry
+    }
+}
+
+template <int LPF, int BPL>
+__device__ __forceinline__ void stage_bonds(const BondArgs& a, FamLoad<BPL>& L)
+{
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        BondSlot& S = L.S[s];
+        const int j = S.act ? L.jr[s] : (L.F.famok ? L.F.fam : 0);
+        const size_t b = S.act ? (size_t)S.b : 0;
+        const double2 Xj = a.X[j], uj = a.u[j];
+        S.x0 = Xj.x - L.F.Xi.x;
+        S.x1 = Xj.y - L.F.Xi.y;
+        S.d0 = uj.x - L.F.ui.x;
+        S.d1 = uj.y - L.F.ui.y;
+        S.Vj = a.V[j];
+        L.br[s] = a.broken[b];
+        L.rv[s] = a.rev[b];
+        L.kinv[s][0] = a.kinv[3 * b];
+        L.kinv[s][1] = a.kinv[3 * b + 1];
+        L.kinv[s][2] = a.kinv[3 * b + 2];
+    }
+}
+
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
+};
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// The two contractions are dense per-family products, D = W P and S = W Q with
+// the symmetric weight matrix W (M x M) and P, Q (M x 4, one 2x2 tensor per
+// bond).  MF = 0 forms them with VALU FMAs; MF = 1 hands them to the matrix
+// cores while the VALU evaluates the weights:
+//  * FP32, v_mfma_f32_4x4x1 (16 blocks of 4 lanes, C_b[r][t] += A[4b+r] B[4b+t]):
+//    A = component (lane & 3) of record l, B = the lane's own weight w(k, l), so
+//    lane k accumulates all four components of its own D_k / S_k.  The tile
+//    layout is unchanged.
+//  * FP64, v_mfma_f64_4x4x4 (4 blocks of 4x4x4; A[p][i][k] in lane 16k+4p+i,
+//    B[p][k][j] in lane 16k+4p+j, C[p][i][j] in lane 16i+4p+j): one step takes
+//    16 bonds kb = 16s + (lane & 15) against 4 neighbours l = 4q + (lane >> 4),
+//    with A = component (lane & 3) of record l and B = w(kb, l); lane holds
+//    component (lane >> 4) of D_kb.  Families are taken one after the other and
+//    the results go back to the bond-per-lane layout through LDS.
+template <typename R, int MF> struct Contraction {
+    static constexpr bool MF32 = MF && sizeof(R) == 4;
+    static constexpr bool MF64 = MF && sizeof(R) == 8;
+};
+
+// One explicit force evaluation of a family group: bond breaking, K
+// re-assembly for families that lost a bond, then the two family contractions
+// and the force-state outputs.  hook_a runs halfway through contraction 1,
+// hook_b after it (a persistent wave issues its next group's loads there).
+template <typename R, int LPF, int BPL, int N2_IS_10, int MF, typename HA, typename HB>
+__device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, double* rec64,
+                                          int lane, const HA& hook_a, const HB& hook_b)
+{
+    constexpr bool MF32 = Contraction<R, MF>::MF32;
+    constexpr bool MF64 = Contraction<R, MF>::MF64;
+    static_assert(!MF64 || (BPL == 1 && (LPF == 32 || LPF == 64)),
+                  "FP64 matrix-core contraction: tiles (32,1) and (64,1)");
+    typedef typename Vec2<R>::T R2;
+    constexpr int FS = Tile<LPF, BPL>::FS;
+    constexpr int FPW = Tile<LPF, BPL>::FPW;
+    const LaneFam& F = L.F;
+    BondSlot (&S)[BPL] = L.S;
+    bool alive[BPL];
+    int nnew = 0;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        // ---- bond breaking (FP64, irreversible, both directions agree) ----
+        alive[s] = S[s].act && L.br[s] == 0;
+        bool newly = false;
+        if (a.s0 > 0.0 && alive[s] && bond_breaks(S[s].x0, S[s].x1, S[s].d0, S[s].d1, a.brk_c)) {
+            alive[s] = false;
+            newly = true;
+            a.broken[S[s].b] = 1;
+        }
+        nnew += __popcll(__ballot(newly));
+    }
+    if (lane == 0 && nnew) atomicAdd(a.counters, (unsigned int)nnew);
+
+    // ---- K_ik^-1: stored, or re-assembled in FP64 when the family changed ----
+    double kinv[BPL][3];
+    if (nnew != 0 || a.force_k) {
+        Geo<double> g64[BPL];
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) g64[s] = geometry64(a, S[s]);
+        bool sing[BPL];
+        assemble_kinv<LPF, BPL, N2_IS_10>(rec64, F, S, alive, g64, a.n2, a.ktol, kinv, sing);
+        int nsing = 0;
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) {
+            if (S[s].act) {
+                a.kinv[3 * (size_t)S[s].b] = kinv[s][0];
+                a.kinv[3 * (size_t)S[s].b + 1] = kinv[s][1];
+                a.kinv[3 * (size_t)S[s].b + 2] = kinv[s][2];
+            }
+            nsing += __popcll(__ballot(sing[s]));
+        }
+        if (lane == 0 && nsing) atomicAdd(a.counters + 1, (unsigned int)nsing);
+    } else {
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) {
+            kinv[s][0] = S[s].act ? L.kinv[s][0] : 0.0;
+            kinv[s][1] = S[s].act ? L.kinv[s][1] : 0.0;
+            kinv[s][2] = S[s].act ? L.kinv[s][2] : 0.0;
+        }
+    }
+
+    // ---- records for the first contraction ----
+    // The VALU FP64 contraction sends V l du and reuses the record's unit vector
+    // (P = (V l du) (x) e, 48 bytes per record instead of 64: that loop is bound
+    // by LDS bytes); the others send the four products.
+    constexpr bool COMPACT = sizeof(R) == 8 && !MF;
+    R* rec = reinterpret_cast<R*>(__builtin_assume_aligned(rec64, 16));
+    Geo<R> g[BPL];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        g[s] = GeoOf<R>::get(a, S[s]);
+        const int k = S[s].k;
+        rec[rec_at<R, FPW>(k, F.slot, 0)] = g[s].ex;
+        rec[rec_at<R, FPW>(k, F.slot, 1)] = g[s].ey;
+        rec[rec_at<R, FPW>(k, F.slot, 2)] = g[s].pe;
+        rec[rec_at<R, FPW>(k, F.slot, 3)] = g[s].qe;
+        if constexpr (COMPACT) {
+            const double Vl = alive[s] ? S[s].Vj * double(g[s].len) : 0.0;
+            rec[rec_at<R, FPW>(k, F.slot, 4)] = R(Vl * S[s].d0);
+            rec[rec_at<R, FPW>(k, F.slot, 5)] = R(Vl * S[s].d1);
+        } else {
+            const R x0r = R(S[s].x0), x1r = R(S[s].x1);
+            const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) : R(0);
+            const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) : R(0);
+            rec[rec_at<R, FPW>(k, F.slot, 4)] = Vd0 * x0r;
+            rec[rec_at<R, FPW>(k, F.slot, 5)] = Vd0 * x1r;
+            rec[rec_at<R, FPW>(k, F.slot, 6)] = Vd1 * x0r;
+            rec[rec_at<R, FPW>(k, F.slot, 7)] = Vd1 * x1r;
+        }
+    }
+    // records past a family's end are zero (inactive slots), so the loops may
+    // run to the wave's largest family rounded up to 4 without guards
+    const int mmax = wave_max_family<LPF>(F.M);
+    wave_sync();
+
+    // ---- contraction 1: D_ik = sum_l w(k,l) V_l du_l (x) xi_l ----
+    const R n2r = R(a.n2);
+    // FP64 matrix-core layout: NS sets of 16 bonds, NQ steps of 4 neighbours
+    constexpr int NS = MF64 ? FS / 16 : 1;
+    constexpr int NQ = MF64 ? FS / 4 : 1;
+    constexpr int WF = MF64 ? FPW : 1;
+    const int lk = lane & 15, lq = lane >> 4, lc = lane & 3;
+    double wc[WF][NQ][NS];   // FP64 matrix-core weight cache
+    int Mf[WF];              // family sizes (wave-uniform)
+    R w[MF64 ? 1 : BPL][MF64 ? 1 : FS];  // per-lane weight rows otherwise
+    R D[BPL][4];
+    v4f Dv[BPL];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        D[s][0] = D[s][1] = D[s][2] = D[s][3] = R(0);
+        Dv[s] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+    if constexpr (MF64) {
+        double Dacc[WF][NS];
+#pragma unroll
+        for (int f = 0; f < WF; ++f) {
+            Mf[f] = __builtin_amdgcn_readfirstlane(__shfl(F.M, f * LPF, 64));
+            double hk[NS], yk[NS], pk[NS], qk[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int kb = 16 * s + lk;
+                hk[s] = 0.5 * rec[rec_at<R, FPW>(kb, f, 0)];
+                yk[s] = 0.5 * rec[rec_at<R, FPW>(kb, f, 1)];
+                pk[s] = rec[rec_at<R, FPW>(kb, f, 2)];
+                qk[s] = rec[rec_at<R, FPW>(kb, f, 3)];
+                Dacc[f][s] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                if (4 * q < Mf[f]) {
+                    const int lb = 4 * q + lq;
+                    const double ex = rec[rec_at<R, FPW>(lb, f, 0)];
+                    const double ey = rec[rec_at<R, FPW>(lb, f, 1)];
+                    const double pe = rec[rec_at<R, FPW>(lb, f, 2)];
+                    const double qe = rec[rec_at<R, FPW>(lb, f, 3)];
+                    const double Pa = rec[rec_at<R, FPW>(lb, f, 4 + lc)];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        if (16 * s < Mf[f]) {
+                            const double om = weight<double, N2_IS_10>(hk[s], yk[s], pk[s], qk[s], ex, ey,
+                                                                       pe, qe, a.n2);
+                            wc[f][q][s] = om;
+                            Dacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(Pa, om, Dacc[f][s], 0, 0, 0);
+                        } else {
+                            wc[f][q][s] = 0.0;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) wc[f][q][s] = 0.0;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                if (f == WF - 1 && q == NQ / 2 - 1) hook_a();
+            }
+        }
+        hook_b();
+        // back to one bond per lane: lane holds component lq of bond 16s + lk
+        wave_sync();
+#pragma unroll
+        for (int f = 0; f < WF; ++f)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) rec[rec_at<R, FPW>(16 * s + lk, f, 4 + lq)] = Dacc[f][s];
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
+    } else {
+#pragma unroll
+    for (int l0 = 0; l0 < FS; l0 += 4) {
+        if (l0 < mmax) {
+#pragma unroll
+            for (int l = l0; l < l0 + 4; ++l) {
+                const R ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
+                const R ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
+                const R pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
+                const R qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
+                if constexpr (MF32) {
+                    const R Pa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s) {
+                        const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        w[s][l] = om;
+                        Dv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Pa, om, Dv[s], 0, 0, 0);
+                    }
+                } else if constexpr (COMPACT) {
+                    const R a0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
+                    const R a1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s) {
+                        const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        w[s][l] = om;
+                        const R t0 = om * a0, t1 = om * a1;
+                        D[s][0] += t0 * ex;
+                        D[s][1] += t0 * ey;
+                        D[s][2] += t1 * ex;
+                        D[s][3] += t1 * ey;
+                    }
+                } else {
+                    const R P0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
+                    const R P1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
+                    const R P2 = rec[rec_at<R, FPW>(l, F.slot, 6)];
+                    const R P3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s) {
+                        const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        w[s][l] = om;
+                        D[s][0] += om * P0;
+                        D[s][1] += om * P1;
+                        D[s][2] += om * P2;
+                        D[s][3] += om * P3;
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int l = l0; l < l0 + 4; ++l)
+#pragma unroll
+                for (int s = 0; s < BPL; ++s) w[s][l] = R(0);
+        }
+        // keep the scheduler from hoisting every record load of the unrolled
+        // loop (which would spill)
+        __builtin_amdgcn_sched_barrier(0);
+        if (l0 == FS / 2 - 4) hook_a();
+    }
+    hook_b();
+    if constexpr (MF32) {
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) D[s][c] = Dv[s][c];
+    }
+    }
+
+    // ---- F - I = D K^-1, plane-stress sigma, Q_k = V_k sigma K^-1 ----
+    const double pf = F.pf;
+    const R c11 = R(pf), c12 = R(pf * a.nu), c33 = R(pf * (1.0 - a.nu));
+    R Q[BPL][4];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        const R K00 = R(kinv[s][0]), K01 = R(kinv[s][1]), K11 = R(kinv[s][2]);
+        const R G00 = D[s][0] * K00 + D[s][1] * K01;
+        const R G01 = D[s][0] * K01 + D[s][1] * K11;
+        const R G10 = D[s][2] * K00 + D[s][3] * K01;
+        const R G11 = D[s][2] * K01 + D[s][3] * K11;
+        const R e11 = G00, e22 = G11, e12 = R(0.5) * (G01 + G10);
+        const R s11 = c11 * e11 + c12 * e22;
+        const R s22 = c12 * e11 + c11 * e22;
+        const R s12 = c33 * e12;
+        const R Vk = alive[s] ? R(S[s].Vj) : R(0);
+        Q[s][0] = (s11 * K00 + s12 * K01) * Vk;
+        Q[s][1] = (s11 * K01 + s12 * K11) * Vk;
+        Q[s][2] = (s12 * K00 + s22 * K01) * Vk;
+        Q[s][3] = (s12 * K01 + s22 * K11) * Vk;
+    }
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        const int k = S[s].k;
+        rec[rec_at<R, FPW>(k, F.slot, 4)] = Q[s][0];
+        rec[rec_at<R, FPW>(k, F.slot, 5)] = Q[s][1];
+        rec[rec_at<R, FPW>(k, F.slot, 6)] = Q[s][2];
+        rec[rec_at<R, FPW>(k, F.slot, 7)] = Q[s][3];
+    }
+    wave_sync();
+
+    // ---- contraction 2: S_ij = sum_k w(j,k) V_k sigma_ik K_ik^-1 ----
+    R Sm[BPL][4];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) Sm[s][0] = Sm[s][1] = Sm[s][2] = Sm[s][3] = R(0);
+    if constexpr (MF64) {
+        double Sacc[WF][NS];
+#pragma unroll
+        for (int f = 0; f < WF; ++f) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) Sacc[f][s] = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                if (4 * q < Mf[f]) {
+                    const double Qa = rec[rec_at<R, FPW>(4 * q + lq, f, 4 + lc)];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        if (16 * s < Mf[f])
+                            Sacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(Qa, wc[f][q][s], Sacc[f][s], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int f = 0; f < WF; ++f)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) rec[rec_at<R, FPW>(16 * s + lk, f, 4 + lq)] = Sacc[f][s];
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
+    } else {
+    v4f Sv[BPL];
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) Sv[s] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int l0 = 0; l0 < FS; l0 += 4) {
+        if (l0 < mmax) {
+#pragma unroll
+            for (int l = l0; l < l0 + 4; ++l) {
+                if constexpr (MF32) {
+                    const R Qa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s)
+                        Sv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Qa, w[s][l], Sv[s], 0, 0, 0);
+                } else {
+                    const R Q0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
+                    const R Q1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
+                    const R Q2 = rec[rec_at<R, FPW>(l, F.slot, 6)];
+                    const R Q3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s) {
+                        Sm[s][0] += w[s][l] * Q0;
+                        Sm[s][1] += w[s][l] * Q1;
+                        Sm[s][2] += w[s][l] * Q2;
+                        Sm[s][3] += w[s][l] * Q3;
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (MF32) {
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Sm[s][c] = Sv[s][c];
+    }
+    }
+    double vsum = 0.0;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) vsum += alive[s] ? S[s].Vj : 0.0;
+    const double VH = family_sum<LPF>(vsum);
+    // 1 / V_H: FP64 division, or the hardware reciprocal in FP32
+    R ivh;
+    if constexpr (sizeof(R) == 8)
+        ivh = VH > 0.0 ? 1.0 / VH : 0.0;
+    else
+        ivh = VH > 0.0 ? __builtin_amdgcn_rcpf(float(VH)) : 0.0f;
+    R T0[BPL], T1[BPL];
+    double fo0 = 0.0, fo1 = 0.0;
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        T0[s] = R(0);
+        T1[s] = R(0);
+        if (alive[s]) {
+            const R x0r = R(S[s].x0), x1r = R(S[s].x1);
+            T0[s] = (Sm[s][0] * x0r + Sm[s][1] * x1r) * ivh;
+            T1[s] = (Sm[s][2] * x0r + Sm[s][3] * x1r) * ivh;
+        }
+        fo0 += double(T0[s]) * S[s].Vj;
+        fo1 += double(T1[s]) * S[s].Vj;
+    }
+
+    // ---- outputs: own-bond sum and the reverse-slot terms ----
+    fo0 = family_sum<LPF>(fo0);
+    fo1 = family_sum<LPF>(fo1);
+    if (F.famok && F.t == 0) a.fown[F.fam] = make_double2(fo0, fo1);
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        if (S[s].act) {
+            R2 t;
+            t.x = T0[s] * R(F.Vi);
+            t.y = T1[s] * R(F.Vi);
+            reinterpret_cast<R2*>(a.W)[L.rv[s]] = t;
+        }
+    }
+}
+
+// Non-persistent launch: one wave per family group.
+template <typename R, int LPF, int BPL, int N2_IS_10, int MF>
+__global__ void __launch_bounds__(512) k_bond(BondArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
+    FamLoad<BPL> L;
+    stage_rows<LPF, BPL>(a, L, lane, blk * wpb + wib);
+    stage_cols<LPF, BPL>(a, L);
+    stage_bonds<LPF, BPL>(a, L);
+    bond_body<R, LPF, BPL, N2_IS_10, MF>(a, L, smem + wib * Tile<LPF, BPL>::REC * 8, lane, NoHook(),
+                                     NoHook());
+}
+
+// Persistent launch: each block owns a contiguous range of family groups (per
+// XCD after the remap); each of its waves walks the range with stride wpb and
+// issues the next group's three load stages inside the current group's
+// contraction, so the dependent gathers overlap compute.  Every wave leaves
+// the loop when its range is done (no grid-wide wait).
+template <typename R, int LPF, int BPL, int N2_IS_10, int MF>
+__global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
+{
+    constexpr int FPW = Tile<LPF, BPL>::FPW;
+    extern __shared__ __attribute__((aligned(16))) double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wpb = blockDim.x >> 6;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
+    const int ngroups = (a.n + FPW - 1) / FPW;
+    const int per = ngroups / (int)gridDim.x, rem = ngroups % (int)gridDim.x;
+    const int g0 = blk * per + (blk < rem ? blk : rem);
+    const int g1 = g0 + per + (blk < rem ? 1 : 0);
+    double* rec64 = smem + wib * Tile<LPF, BPL>::REC * 8;
+    int g = g0 + wib;
+    if (g >= g1) return;
+    FamLoad<BPL> cur, nxt;
+    stage_rows<LPF, BPL>(a, cur, lane, g);
+    stage_cols<LPF, BPL>(a, cur);
+    stage_bonds<LPF, BPL>(a, cur);
+    for (;;) {
+        const int gn = g + wpb;
+        const bool more = gn < g1;
+        // the last group prefetches itself again (discarded): the loads stay
+        // unconditional, so waits on the current group's data are counted
+        // exactly instead of draining the prefetch
+        const int gp = more ? gn : g;
+        stage_rows<LPF, BPL>(a, nxt, lane, gp);
+        bond_body<R, LPF, BPL, N2_IS_10, MF>(
+            a, cur, rec64, lane,
+            [&]() { stage_cols<LPF, BPL>(a, nxt); },
+            [&]() { stage_bonds<LPF, BPL>(a, nxt); });
+        if (!more) break;
+        cur = nxt;
+        g = gn;
+        wave_sync();
+    }
+}
+
+}  // namespace pd

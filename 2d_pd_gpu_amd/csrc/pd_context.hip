@@ -33,6 +33,7 @@ struct pd_context {
     int maxM = 0;
     int lpf = 32;
     int bpl = 1;
+    int mf = 0;              // family contractions on the matrix cores
     int n2_is_10 = 1;
     int dev = 0;
     std::vector<int> perm;   // internal -> caller
@@ -45,6 +46,7 @@ struct pd_context {
     double2 *X = nullptr, *u = nullptr, *v = nullptr, *a = nullptr, *f = nullptr, *fown = nullptr;
     double2* v0 = nullptr;
     double *V = nullptr, *E = nullptr, *kinv = nullptr;
+    double* pf1 = nullptr;   // the uniform plane-stress factor (one value)
     int *row = nullptr, *col = nullptr, *rev = nullptr, *bc = nullptr, *ids = nullptr;
     long ids_cap = 0;
     unsigned char* broken = nullptr;
@@ -139,7 +141,7 @@ static void free_all(pd_context* c)
 {
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
-    void* ptrs[] = {c->X, c->u, c->v, c->a, c->f, c->fown, c->v0, c->V, c->E, c->kinv,
+    void* ptrs[] = {c->X, c->u, c->v, c->a, c->f, c->fown, c->v0, c->V, c->E, c->pf1, c->kinv,
                     c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->W, c->counters};
     for (void* q : ptrs)
         if (q) hipFree(q);
@@ -165,8 +167,8 @@ static BondArgs bond_args(pd_context* c)
     a.W = c->W;
     a.fown = c->fown;
     // plane-stress factor E/(1-nu^2) of getStiffnessTensor (src/main_seq.cpp:185)
-    a.pf = c->E;
-    a.pf0 = c->p.E / (1.0 - c->p.nu * c->p.nu);
+    a.pf = c->E ? c->E : c->pf1;
+    a.pf_mask = c->E ? ~0 : 0;
     a.brk_c = (1.0 + c->p.s0) * (1.0 + c->p.s0);
     a.nu = c->p.nu;
     a.A = c->p.n1 / (c->p.horizon * c->p.dx);
@@ -426,7 +428,8 @@ extern "C" int pd_create(const pd_params* p, int n, const double* X, const doubl
         (rc = dalloc(c, &c->a, n)) || (rc = dalloc(c, &c->f, n)) || (rc = dalloc(c, &c->fown, n)) ||
         (rc = dalloc(c, &c->V, n)) || (rc = dalloc(c, &c->row, n + 1)) ||
         (rc = dalloc(c, &c->col, nb)) || (rc = dalloc(c, &c->rev, nb)) ||
-        (rc = dalloc(c, &c->kinv, 3 * (size_t)nb)) || (rc = dalloc(c, &c->broken, nb)) ||
+        (rc = dalloc(c, &c->kinv, 3 * (size_t)std::max(nb, 1L))) || (rc = dalloc(c, &c->broken, nb)) ||
+        (rc = dalloc(c, &c->pf1, 1)) ||
         (rc = dalloc(c, (char**)&c->W, wbytes * (size_t)(nb > 0 ? nb : 1))) ||
         (rc = dalloc(c, &c->counters, 4)) || (rc = dalloc(c, &c->bc, n)) ||
         (rc = dalloc(c, &c->v0, 1))) {
@@ -456,6 +459,18 @@ extern "C" int pd_create(const pd_params* p, int n, const double* X, const doubl
     e = e != hipSuccess ? e : hipMemset(c->f, 0, sizeof(double2) * n);
     e = e != hipSuccess ? e : hipMemset(c->fown, 0, sizeof(double2) * n);
     e = e != hipSuccess ? e : hipMemset(c->counters, 0, 16);
+    e = e != hipSuccess ? e : hipMemset(c->kinv, 0, sizeof(double) * 3 * std::max(nb, 1L));
+    if (nb == 0) {
+        // the bond kernel reads entry 0 of the bond arrays for idle lanes
+        e = e != hipSuccess ? e : hipMemset(c->col, 0, sizeof(int));
+        e = e != hipSuccess ? e : hipMemset(c->rev, 0, sizeof(int));
+        e = e != hipSuccess ? e : hipMemset(c->broken, 0, 1);
+    }
+    {
+        // plane-stress factor E/(1-nu^2) of getStiffnessTensor (src/main_seq.cpp:185)
+        const double pf = c->p.E / (1.0 - c->p.nu * c->p.nu);
+        e = e != hipSuccess ? e : hipMemcpy(c->pf1, &pf, sizeof(double), hipMemcpyHostToDevice);
+    }
     e = e != hipSuccess ? e : hipDeviceSynchronize();
     if (e != hipSuccess) return bail(fail(nullptr, PD_ERR_HIP, "upload failed: %s", hipGetErrorString(e)));
     {
@@ -547,16 +562,18 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
         c->persist = (int)value;
         break;
     case PD_OPT_TILE: {
-        int lpf = (int)(value / 100), bpl = (int)(value % 100);
+        int mf = (int)(value / 10000), lpf = (int)(value % 10000 / 100), bpl = (int)(value % 100);
         if (value == 0) {
             lpf = c->maxM <= 32 ? 32 : 64;
             bpl = 1;
         }
-        if (!tile_supported(c->p.precision == PD_MIXED, lpf, bpl) || lpf * bpl < c->maxM)
-            return fail(c, PD_ERR_INVALID, "tile %ld (lanes %d x bonds %d) unsupported for this precision "
-                        "or smaller than the largest family (%d)", value, lpf, bpl, c->maxM);
+        if (value < 0 || mf > 1 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
+            lpf * bpl < c->maxM)
+            return fail(c, PD_ERR_INVALID, "tile %ld (lanes %d x bonds %d, matrix cores %d) unsupported for "
+                        "this precision or smaller than the largest family (%d)", value, lpf, bpl, mf, c->maxM);
         c->lpf = lpf;
         c->bpl = bpl;
+        c->mf = mf;
         break;
     }
     default: return fail(c, PD_ERR_INVALID, "unknown option %d", opt);
@@ -687,7 +704,7 @@ static int force_and_velocity(pd_context* c, int write_a)
     BondArgs a = bond_args(c);
     GatherArgs g = gather_args(c, write_a);
     int mixed = c->p.precision == PD_MIXED;
-    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->n2_is_10, c->wpb, c->persist * c->n_cu, c->stream); }))
+    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->mf, c->n2_is_10, c->wpb, c->persist * c->n_cu, c->stream); }))
         return rc;
     return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream); });
 }
@@ -830,6 +847,7 @@ extern "C" int pd_get_stats(pd_context* c, pd_stats* s)
     s->max_family = c->maxM;
     s->lanes_per_family = c->lpf;
     s->bonds_per_lane = c->bpl;
+    s->matrix_cores = c->mf;
     s->persistent = c->persist;
     s->waves_per_wg = c->wpb;
     s->steps = c->steps;
@@ -904,24 +922,26 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     const int saved_profile = c->profile;
     const long saved_steps = c->steps;
     c->profile = 0;
-    struct Var { int wpb, xcd, lpf, bpl, persist; };
+    struct Var { int wpb, xcd, lpf, bpl, mf, persist; };
     std::vector<Var> vars;
     const int mixed = c->p.precision == PD_MIXED;
     const int tiles[4][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}};
-    for (const auto& tl : tiles) {
-        if (!tile_supported(mixed, tl[0], tl[1]) || tl[0] * tl[1] < c->maxM) continue;
-        for (int w : {1, 2, 4, 8})
-            for (int x : {1, 8})
-                for (int pz : {0, 1, 2, 4})
-                    if (pz == 0 || w * pz <= 32) vars.push_back({w, x, tl[0], tl[1], pz});
-    }
+    for (int mf : {0, 1})
+        for (const auto& tl : tiles) {
+            if (!tile_supported(mixed, tl[0], tl[1], mf) || tl[0] * tl[1] < c->maxM) continue;
+            for (int w : {1, 2, 4, 8})
+                for (int x : {1, 8})
+                    for (int pz : {0, 1, 2, 4})
+                        if (pz == 0 || w * pz <= 32) vars.push_back({w, x, tl[0], tl[1], mf, pz});
+        }
     double best = 1e300;
-    Var bestv = {c->wpb, c->xcd, c->lpf, c->bpl, c->persist};
+    Var bestv = {c->wpb, c->xcd, c->lpf, c->bpl, c->mf, c->persist};
     for (const Var& vv : vars) {
         c->wpb = vv.wpb;
         c->xcd = vv.xcd;
         c->lpf = vv.lpf;
         c->bpl = vv.bpl;
+        c->mf = vv.mf;
         c->persist = vv.persist;
         c->graph_valid = false;
         if ((rc = pd_step(c, 1))) break;  // warm-up
@@ -944,6 +964,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     c->xcd = bestv.xcd;
     c->lpf = bestv.lpf;
     c->bpl = bestv.bpl;
+    c->mf = bestv.mf;
     c->persist = bestv.persist;
     c->graph_valid = false;
     c->profile = saved_profile;

@@ -23,8 +23,9 @@ struct BondArgs {
     double* kinv;             // per bond K_ik^-1 (xx, xy, yy), FP64
     void* W;                  // per bond, slot rev[b] <- T_ij V_i (double2 / float2)
     double2* fown;            // per particle sum_j T_ij V_j
-    const double* pf;         // per particle plane-stress factor E/(1-nu^2), or nullptr
-    double pf0, nu;           // uniform E/(1-nu^2), Poisson ratio
+    const double* pf;         // plane-stress factor E/(1-nu^2): pf[i & pf_mask]
+    int pf_mask;              // ~0: one value per particle; 0: one value for all
+    double nu;                // Poisson ratio
     double A;                 // n1 / (horizon * dx)
     double n2;                // angular exponent
     double s0;                // critical stretch (<= 0: none)
@@ -60,11 +61,17 @@ struct PosArgs {
 
 // Tiles: (lanes per family, bonds per lane) in {(32,1), (64,1), (16,2)} and,
 // for mixed precision, (32,2); a family may hold up to lpf * bpl bonds.
-bool tile_supported(int mixed, int lpf, int bpl);
+// mf = 1: family contractions on the matrix cores (every mixed tile; FP64
+// tiles (32,1) and (64,1)).
+bool tile_supported(int mixed, int lpf, int bpl, int mf);
 // pgrid > 0: persistent launch with pgrid workgroups (each walks a contiguous
 // range of family groups, prefetching the next group's loads).
-hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int n2_is_10, int wpb,
+hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int mf, int n2_is_10, int wpb,
                        int pgrid, hipStream_t s);
+hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int n2_is_10, int wpb,
+                           int pgrid, hipStream_t s);
+hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int n2_is_10, int wpb,
+                           int pgrid, hipStream_t s);
 hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int wpb, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, int mixed, hipStream_t s);
 hipError_t launch_positions(const PosArgs& a, hipStream_t s);

@@ -84,6 +84,11 @@ def cpu_baseline(pkg, w, state, budget_s=12.0):
     }
 
 
+def tile_info(t):
+    return {"lanes_per_family": t[0], "bonds_per_lane": t[1], "matrix_cores": bool(t[2]),
+            "waves_per_wg": t[3], "persistent_wg_per_cu": t[4]}
+
+
 def read_traffic(workload, prec):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{prec}.json")
     if not os.path.exists(path):
@@ -165,7 +170,8 @@ def main():
         kb, sb = kernel_bytes(w.n, w.nb, prec == pkg.MIXED)
         res[name] = dict(t=t_graph, t_prof=t_prof, nb_alive=nb_alive, bond_ms=bond_ms,
                          gather_ms=gather_ms, pos_ms=pos_ms, kb=kb, sb=sb, state=state,
-                         n_broken=st.n_broken, tile=(st.lanes_per_family, st.bonds_per_lane))
+                         n_broken=st.n_broken, tile=(st.lanes_per_family, st.bonds_per_lane, st.matrix_cores,
+                               st.waves_per_wg, st.persistent))
         print(f"# {name}: {args.steps} steps graph {t_graph:.3f}s eager+events {t_prof:.3f}s "
               f"bond {bond_ms:.4f} ms gather {gather_ms:.4f} ms positions {pos_ms:.4f} ms "
               f"broken {st.n_broken}", file=sys.stderr, flush=True)
@@ -195,7 +201,7 @@ def main():
         "step_hbm_frac": r["sb"] / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "copy_probe_gbs": copy_gbs,
         "kernel_ms": {"bond": r["bond_ms"], "gather": r["gather_ms"], "positions": r["pos_ms"]},
-        "tile": {"lanes_per_family": r["tile"][0], "bonds_per_lane": r["tile"][1]},
+        "tile": tile_info(r["tile"]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "kernel": "k_bond<double>",
                      "traffic": read_traffic(w.name, "fp64")},
@@ -215,7 +221,7 @@ def main():
             "particle_steps_per_s": w.n * args.steps / m["t"],
             "step_hbm_frac": m["sb"] / (ms_m * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "kernel_ms": {"bond": m["bond_ms"], "gather": m["gather_ms"], "positions": m["pos_ms"]},
-            "tile": {"lanes_per_family": m["tile"][0], "bonds_per_lane": m["tile"][1]},
+            "tile": tile_info(m["tile"]),
             "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach_m / HBM_PEAK_GBS, "kernel": "k_bond<float>",
                          "traffic": read_traffic(w.name, "mixed")},

@@ -45,10 +45,12 @@ def hassa_oracle(hassa_case):
     return u, v, a, f, T
 
 
-def hassa_gpu(pdpkg, hassa_case, via="phases", precision=0):
+def hassa_gpu(pdpkg, hassa_case, via="phases", precision=0, tile=None):
     pdh = pdpkg.pdhip
     X, V, row, col, sets = hassa_case
     ctx = pdh.Context(hassa_params(pdh, precision), X.astype(float), V.astype(float), row, col)
+    if tile is not None:
+        ctx.set_option(pdh.OPT_TILE, tile)
     if via == "phases":
         # src/main_seq.cpp:686-699
         ctx.apply_velocity_bc(sets[3], H.V_TOP)
@@ -197,8 +199,10 @@ def run_plate_oracle(w, s0, steps, E=None):
     return u, v, f, br
 
 
-def run_plate_gpu(pdh, w, s0, steps, precision=0, E=None, chunks=1, graph=True):
+def run_plate_gpu(pdh, w, s0, steps, precision=0, E=None, chunks=1, graph=True, tile=None):
     ctx = w.context(precision=precision, s0=s0)
+    if tile is not None:
+        ctx.set_option(pdh.OPT_TILE, tile)
     if E is not None:
         ctx.set_youngs_modulus(E)
     ctx.set_option(pdh.OPT_USE_GRAPH, int(graph))
@@ -264,7 +268,7 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     pdh = pdpkg.pdhip
     ref, bref, _ = run_plate_gpu(pdh, small_plate, 0.005, 40)
     mask = small_plate.code >= 0
-    for tile in (3201, 6401, 1602):
+    for tile in (3201, 6401, 1602, 13201, 16401):
         tile_ref = None
         for wpb, xcd, persist in ((1, 0, 0), (2, 1, 0), (8, 0, 0), (8, 1, 0), (4, 1, 1), (2, 0, 3),
                                   (8, 1, 2)):
@@ -282,11 +286,12 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
                 if tile_ref is None:
                     tile_ref = v
                 np.testing.assert_array_equal(v, tile_ref)
-        # two bonds per lane reorder the FP64 family sums: round-off only
+        # two bonds per lane and the matrix-core contractions reorder the FP64
+        # family sums: round-off only
         _, nonbc, _ = H.parity_norms(tile_ref, ref["v"], mask)
         assert nonbc < 1e-13
     mref = None
-    for tile in (3201, 6401, 1602, 3202):
+    for tile in (3201, 6401, 1602, 3202, 13201, 16401, 11602, 13202):
         ctx = small_plate.context(precision=1, s0=0.005)
         ctx.set_option(pdh.OPT_TILE, tile)
         ctx.step(40)
@@ -302,6 +307,12 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
             ctx.set_option(pdh.OPT_TILE, 3202)  # FP64 has no (32, 2) tile
         finally:
             ctx.close()
+    with pytest.raises(pdh.PDError):
+        ctx = small_plate.context(s0=0.005)
+        try:
+            ctx.set_option(pdh.OPT_TILE, 11602)  # nor a matrix-core (16, 2) tile
+        finally:
+            ctx.close()
     ctx = small_plate.context(s0=0.005)
     ctx.autotune(3)
     st = ctx.stats()
@@ -313,16 +324,52 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     ctx.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_persistent_bond_kernel_bitwise(pdpkg, precision):
+def test_hassa_matrix_core_contractions(pdpkg, hassa_case, hassa_oracle):
+    """Family contractions on the matrix cores (v_mfma_f64_4x4x4 in FP64,
+    v_mfma_f32_4x4x1 in mixed precision) meet the same bars as the VALU path."""
+    say("hassa single step, matrix-core contractions")
+    u, v, a, f, T = hassa_oracle
+    for tile in (13201, 16401):
+        st = hassa_gpu(pdpkg, hassa_case, tile=tile)
+        np.testing.assert_array_equal(st["u"], u)
+        _, nonbc, elem = H.parity_norms(st["v"], v, bc_mask(hassa_case))
+        assert nonbc < 1e-12 and elem < 1e-12
+        assert np.linalg.norm(st["T"] - T) < 1e-12 * np.linalg.norm(T)
+    for tile in (13201, 16401, 11602, 13202):
+        st = hassa_gpu(pdpkg, hassa_case, precision=1, tile=tile)
+        _, nonbc, _ = H.parity_norms(st["v"], v, bc_mask(hassa_case))
+        assert nonbc < 1e-5
+
+
+@pytest.mark.parametrize("tile", [13201, 16401])
+def test_plate_matrix_core_breaking_matches_oracle(pdpkg, small_plate, small_plate_oracle, tile):
+    say(f"plate 48x40, 300 steps, s0 = 0.005, matrix-core tile {tile}")
+    pdh = pdpkg.pdhip
+    u, v, f, br = small_plate_oracle
+    mask = small_plate.code >= 0
+    st, brg, stats = run_plate_gpu(pdh, small_plate, 0.005, 300, tile=tile)
+    assert stats.matrix_cores == 1
+    np.testing.assert_array_equal(brg, br)
+    _, nonbc, _ = H.parity_norms(st["v"], v, mask)
+    assert nonbc < 1e-10
+    assert np.linalg.norm(st["u"] - u) < 1e-10 * np.linalg.norm(u)
+    st, brg, _ = run_plate_gpu(pdh, small_plate, 0.005, 300, precision=1, tile=tile)
+    np.testing.assert_array_equal(brg, br)
+    _, nonbc, _ = H.parity_norms(st["v"], v, mask)
+    assert nonbc < 1e-5
+
+
+@pytest.mark.parametrize("precision,tile", [(0, 0), (1, 0), (0, 13201), (1, 13201)])
+def test_persistent_bond_kernel_bitwise(pdpkg, precision, tile):
     """The persistent bond kernel (each wave walks many family groups and
     prefetches the next one) gives the same bits as one wave per group."""
-    say(f"persistent bond kernel, 200 x 200 plate, precision {precision}")
+    say(f"persistent bond kernel, 200 x 200 plate, precision {precision}, tile {tile}")
     pdh = pdpkg.pdhip
     w = pdpkg.plates.single_plate("p200", 200, 200, 40.0, notch_len_cells=40, seed=8)
     out = []
     for persist, wpb in ((0, 4), (1, 4), (2, 2), (1, 8)):
         ctx = w.context(precision=precision, s0=0.003)
+        ctx.set_option(pdh.OPT_TILE, tile)
         ctx.set_option(pdh.OPT_PERSISTENT, persist)
         ctx.set_option(pdh.OPT_WAVES_PER_WG, wpb)
         ctx.step(30)
@@ -354,9 +401,13 @@ def test_delta_4015_family_of_48(pdpkg):
     assert stats.lanes_per_family * stats.bonds_per_lane >= 48
     _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
     assert nonbc < 1e-11
-    st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, precision=1)
+    st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, tile=16401)
     _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
-    assert nonbc < 1e-5
+    assert nonbc < 1e-11
+    for tile in (6401, 16401):
+        st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, precision=1, tile=tile)
+        _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
+        assert nonbc < 1e-5
 
 
 def test_batched_plates_equal_individual_runs(pdpkg):
