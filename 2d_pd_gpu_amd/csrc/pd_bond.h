@@ -190,28 +190,51 @@ template <typename R> struct Geo {
     R ex, ey, hx, hy, pe, qe, len;
 };
 
+// FP64 1/sqrt(x) and 1/x for positive normal x: the hardware estimate and two
+// Newton steps (quadratic convergence from ~2^-23 leaves a final error of an
+// ulp or two, far inside the FP64 parity budget), instead of the
+// correctly-rounded sqrt, division and a second exp.
+__device__ __forceinline__ double rsqrt64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    return y;
+}
+
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
 __device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondSlot& S)
 {
     Geo<double> g;
-    const double len = sqrt(S.x0 * S.x0 + S.x1 * S.x1);
-    const double il = S.act ? 1.0 / len : 0.0;
+    const double l2 = S.x0 * S.x0 + S.x1 * S.x1;
+    const double il = S.act ? rsqrt64(l2) : 0.0;
+    const double len = l2 * il;
     g.len = len;
     g.ex = S.x0 * il;
     g.ey = S.x1 * il;
     g.hx = 0.5 * g.ex;
     g.hy = 0.5 * g.ey;
     g.pe = S.act ? exp(a.A * len) : 0.0;
-    g.qe = S.act ? exp(-a.A * len) : 0.0;
+    g.qe = S.act ? rcp64(g.pe) : 0.0;
     return g;
 }
 
-// FP32 geometry for the mixed-precision contractions (hardware exp / rcp).
+// FP32 geometry for the mixed-precision contractions (hardware rsq / exp / rcp).
 __device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSlot& S)
 {
     Geo<float> g;
     const float x0 = float(S.x0), x1 = float(S.x1);
-    const float len = sqrtf(x0 * x0 + x1 * x1);
-    const float il = S.act ? __builtin_amdgcn_rcpf(len) : 0.0f;
+    const float l2 = x0 * x0 + x1 * x1;
+    const float il = S.act ? __builtin_amdgcn_rsqf(l2) : 0.0f;
+    const float len = l2 * il;
     g.len = len;
     g.ex = x0 * il;
     g.ey = x1 * il;
@@ -219,7 +242,7 @@ __device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSl
     g.hy = 0.5f * g.ey;
     const float Al = float(a.A) * len;
     g.pe = S.act ? __expf(Al) : 0.0f;
-    g.qe = S.act ? __expf(-Al) : 0.0f;
+    g.qe = S.act ? __builtin_amdgcn_rcpf(g.pe) : 0.0f;
     return g;
 }
 
