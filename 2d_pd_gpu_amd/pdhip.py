@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "lib2dpdgpu.so")
+# the in-tree build; PD_HIP_LIB names another build of the same library
+LIB_PATH = os.environ.get("PD_HIP_LIB") or os.path.join(_HERE, "lib", "lib2dpdgpu.so")
 
 PD_OK = 0
 PD_ERR_INVALID = 1

@@ -97,10 +97,13 @@ def read_traffic(workload, prec):
         return json.load(fh).get("bytes_per_launch")
 
 
-def run(pkg, w, precision, steps, warmup, tune):
+def run(pkg, w, precision, steps, warmup, tune, tile=None, wpb=4):
     pdh = pkg.pdhip
     ctx = w.context(precision=precision)
-    if tune:
+    if tile is not None:
+        ctx.set_option(pdh.OPT_TILE, tile)
+        ctx.set_option(pdh.OPT_WAVES_PER_WG, wpb)
+    elif tune:
         # pick the family tile / workgroup shape on this plate (state restored)
         t0 = time.perf_counter()
         ctx.autotune(tune)
@@ -132,6 +135,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="plate_1024")
     ap.add_argument("--tune", type=int, default=3, help="autotuner steps per variant (0: off)")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="pin the family tile (PD_OPT_TILE value) instead of autotuning")
+    ap.add_argument("--wpb", type=int, default=4, help="waves per workgroup with --tile")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
@@ -161,7 +167,8 @@ def main():
             continue
         if args.only and name != args.only:
             continue
-        t_graph, t_prof, st, state = run(pkg, w, prec, args.steps, args.warmup, args.tune)
+        t_graph, t_prof, st, state = run(pkg, w, prec, args.steps, args.warmup, args.tune, args.tile,
+                                         args.wpb)
         nb_alive = w.nb - st.n_broken
         calls = max(1, st.phase_calls[1])
         bond_ms = st.phase_ms[1] / calls
