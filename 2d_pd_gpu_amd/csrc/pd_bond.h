@@ -59,6 +59,42 @@ __device__ __forceinline__ bool bond_breaks(double x0, double x1, double d0, dou
     return cur2 > c * len2;
 }
 
+// Mazars bond damage for one bond's strain (e11, e22, e12): restatement of
+// computeDamageStatus (src/main_seq.cpp:473-598).  Closed-form principal strains
+// l1 <= l2 replace Eigen's SelfAdjointEigenSolver (:523-526); eps_eq =
+// sqrt(sum <l>+^2) (:529-536); past the threshold (:540) the tension and
+// compression parts share eps's principal directions, so their ascending
+// spectra are <l>+ and <l>- (the reference accumulates them into uninitialised
+// matrices, :509, :513; here they start at zero), giving alpha_t, alpha_c
+// (:571-578) and d = alpha_t d_t + alpha_c d_c (:581-584).  Mode 1 puts the
+// running threshold into the law as :581-582 do; mode 2 the initial kappa0.
+// Returns true when kappa and d changed.
+__device__ __forceinline__ bool mazars_update(const BondArgs& a, double e11, double e22, double e12,
+                                              double& kappa, double& d)
+{
+    const double m = 0.5 * (e11 + e22);
+    const double h = 0.5 * (e11 - e22);
+    const double r = sqrt(h * h + e12 * e12);
+    const double l1 = m - r, l2 = m + r;
+    const double t1 = l1 > 0.0 ? l1 : 0.0, t2 = l2 > 0.0 ? l2 : 0.0;
+    const double c1 = l1 < 0.0 ? l1 : 0.0, c2 = l2 < 0.0 ? l2 : 0.0;
+    const double s2 = t1 * t1 + t2 * t2;
+    const double eq = s2 > 0.0 ? sqrt(s2) : 0.0;
+    if (!(eq > kappa)) return false;
+    const double e2 = eq * eq;
+    double at = 0.0, ac = 0.0;
+    if (t1 > 0.0) at += t1 * (t1 + c1) / e2;
+    if (t2 > 0.0) at += t2 * (t2 + c2) / e2;
+    if (c1 > 0.0) ac += c1 * (t1 + c1) / e2;
+    if (c2 > 0.0) ac += c2 * (t2 + c2) / e2;
+    const double kr = a.dmg_mode == 2 ? a.kap0 : kappa;
+    const double dt = 1.0 - kr * (1.0 - a.At) / eq - a.At / exp(a.Bt * (eq - kr));
+    const double dc = 1.0 - kr * (1.0 - a.Ac) / eq - a.Ac / exp(a.Bc * (eq - kr));
+    d = at * dt + ac * dc;
+    kappa = eq;
+    return true;
+}
+
 __device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
 __device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ double rmax(double a, double b) { return fmax(a, b); }
@@ -370,6 +406,7 @@ template <int BPL> struct FamLoad {
     int rv[BPL];              // reverse bond
     unsigned char br[BPL];
     double kinv[BPL][3];
+    double kap[BPL], dm[BPL];  // damage threshold and value (damage kernels)
 };
 
 template <int LPF, int BPL>
@@ -392,7 +429,7 @@ __device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<BPL>& L)
     }
 }
 
-template <int LPF, int BPL>
+template <int LPF, int BPL, int DMG = 0>
 __device__ __forceinline__ void stage_bonds(const BondArgs& a, FamLoad<BPL>& L)
 {
 #pragma unroll
@@ -411,6 +448,10 @@ __device__ __forceinline__ void stage_bonds(const BondArgs& a, FamLoad<BPL>& L)
         L.kinv[s][0] = a.kinv[3 * b];
         L.kinv[s][1] = a.kinv[3 * b + 1];
         L.kinv[s][2] = a.kinv[3 * b + 2];
+        if constexpr (DMG) {
+            L.kap[s] = a.kappa[b];
+            L.dm[s] = a.dmg[b];
+        }
     }
 }
 
@@ -443,7 +484,7 @@ template <typename R, int MF> struct Contraction {
 // re-assembly for families that lost a bond, then the two family contractions
 // and the force-state outputs.  hook_a runs halfway through contraction 1,
 // hook_b after it (a persistent wave issues its next group's loads there).
-template <typename R, int LPF, int BPL, int N2_IS_10, int MF, typename HA, typename HB>
+template <typename R, int LPF, int BPL, int N2_IS_10, int MF, int DMG, typename HA, typename HB>
 __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, double* rec64,
                                           int lane, const HA& hook_a, const HB& hook_b)
 {
@@ -677,6 +718,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
     const double pf = F.pf;
     const R c11 = R(pf), c12 = R(pf * a.nu), c33 = R(pf * (1.0 - a.nu));
     R Q[BPL][4];
+    double dsum = 0.0;
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         const R K00 = R(kinv[s][0]), K01 = R(kinv[s][1]), K11 = R(kinv[s][2]);
@@ -685,6 +727,26 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
         const R G10 = D[s][2] * K00 + D[s][3] * K01;
         const R G11 = D[s][2] * K01 + D[s][3] * K11;
         const R e11 = G00, e22 = G11, e12 = R(0.5) * (G01 + G10);
+        if constexpr (DMG) {
+            // a broken bond counts as fully damaged; a singular K (K^-1 = 0)
+            // gives zero strain, so its bond is left as it was
+            double d = L.dm[s];
+            if (S[s].act) {
+                if (!alive[s]) {
+                    if (d != 1.0) {
+                        d = 1.0;
+                        a.dmg[S[s].b] = 1.0;
+                    }
+                } else {
+                    double kap = L.kap[s];
+                    if (mazars_update(a, double(e11), double(e22), double(e12), kap, d)) {
+                        a.kappa[S[s].b] = kap;
+                        a.dmg[S[s].b] = d;
+                    }
+                }
+                dsum += d;
+            }
+        }
         const R s11 = c11 * e11 + c12 * e22;
         const R s22 = c12 * e11 + c11 * e22;
         const R s12 = c33 * e12;
@@ -804,6 +866,11 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
     fo0 = family_sum<LPF>(fo0);
     fo1 = family_sum<LPF>(fo1);
     if (F.famok && F.t == 0) a.fown[F.fam] = make_double2(fo0, fo1);
+    if constexpr (DMG) {
+        // damageStatus_i = sum_k d_ik / |H_i| (src/main_seq.cpp:593)
+        dsum = family_sum<LPF>(dsum);
+        if (F.famok && F.t == 0) a.dstat[F.fam] = F.M > 0 ? dsum / F.M : 0.0;
+    }
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         if (S[s].act) {
@@ -816,7 +883,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 }
 
 // Non-persistent launch: one wave per family group.
-template <typename R, int LPF, int BPL, int N2_IS_10, int MF>
+template <typename R, int LPF, int BPL, int N2_IS_10, int MF, int DMG>
 __global__ void __launch_bounds__(512) k_bond(BondArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) double smem[];
@@ -827,9 +894,9 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     FamLoad<BPL> L;
     stage_rows<LPF, BPL>(a, L, lane, blk * wpb + wib);
     stage_cols<LPF, BPL>(a, L);
-    stage_bonds<LPF, BPL>(a, L);
-    bond_body<R, LPF, BPL, N2_IS_10, MF>(a, L, smem + wib * Tile<LPF, BPL>::REC * 8, lane, NoHook(),
-                                     NoHook());
+    stage_bonds<LPF, BPL, DMG>(a, L);
+    bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * Tile<LPF, BPL>::REC * 8, lane,
+                                              NoHook(), NoHook());
 }
 
 // Persistent launch: each block owns a contiguous range of family groups (per
@@ -837,7 +904,7 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
 // issues the next group's three load stages inside the current group's
 // contraction, so the dependent gathers overlap compute.  Every wave leaves
 // the loop when its range is done (no grid-wide wait).
-template <typename R, int LPF, int BPL, int N2_IS_10, int MF>
+template <typename R, int LPF, int BPL, int N2_IS_10, int MF, int DMG>
 __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
 {
     constexpr int FPW = Tile<LPF, BPL>::FPW;
@@ -856,7 +923,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
     FamLoad<BPL> cur, nxt;
     stage_rows<LPF, BPL>(a, cur, lane, g);
     stage_cols<LPF, BPL>(a, cur);
-    stage_bonds<LPF, BPL>(a, cur);
+    stage_bonds<LPF, BPL, DMG>(a, cur);
     for (;;) {
         const int gn = g + wpb;
         const bool more = gn < g1;
@@ -865,10 +932,10 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
         // exactly instead of draining the prefetch
         const int gp = more ? gn : g;
         stage_rows<LPF, BPL>(a, nxt, lane, gp);
-        bond_body<R, LPF, BPL, N2_IS_10, MF>(
+        bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(
             a, cur, rec64, lane,
             [&]() { stage_cols<LPF, BPL>(a, nxt); },
-            [&]() { stage_bonds<LPF, BPL>(a, nxt); });
+            [&]() { stage_bonds<LPF, BPL, DMG>(a, nxt); });
         if (!more) break;
         cur = nxt;
         g = gn;

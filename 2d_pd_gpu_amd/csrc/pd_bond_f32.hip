@@ -1,25 +1,27 @@
 // pd_bond_f32.hip -- mixed-precision bond-kernel variants: tiles (32,1),
-// (64,1), (16,2), (32,2), each with VALU or matrix-core contractions.
+// (64,1), (16,2), (32,2), each with VALU or matrix-core contractions, and
+// with VALU contractions plus the damage update.
 #include "pd_bond_launch.h"
 
 namespace pd {
 
-template <int MF>
+template <int MF, int DMG>
 static hipError_t launch_f32_mf(const BondArgs& a, int lpf, int bpl, int n2_is_10, int wpb, int pgrid,
                                 hipStream_t s)
 {
-    if (bpl == 1 && lpf == 32) return launch_bond_n2<float, 32, 1, MF>(a, n2_is_10, wpb, pgrid, s);
-    if (bpl == 1 && lpf == 64) return launch_bond_n2<float, 64, 1, MF>(a, n2_is_10, wpb, pgrid, s);
-    if (bpl == 2 && lpf == 16) return launch_bond_n2<float, 16, 2, MF>(a, n2_is_10, wpb, pgrid, s);
-    if (bpl == 2 && lpf == 32) return launch_bond_n2<float, 32, 2, MF>(a, n2_is_10, wpb, pgrid, s);
+    if (bpl == 1 && lpf == 32) return launch_bond_n2<float, 32, 1, MF, DMG>(a, n2_is_10, wpb, pgrid, s);
+    if (bpl == 1 && lpf == 64) return launch_bond_n2<float, 64, 1, MF, DMG>(a, n2_is_10, wpb, pgrid, s);
+    if (bpl == 2 && lpf == 16) return launch_bond_n2<float, 16, 2, MF, DMG>(a, n2_is_10, wpb, pgrid, s);
+    if (bpl == 2 && lpf == 32) return launch_bond_n2<float, 32, 2, MF, DMG>(a, n2_is_10, wpb, pgrid, s);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int n2_is_10, int wpb,
+hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s)
 {
-    return mf ? launch_f32_mf<1>(a, lpf, bpl, n2_is_10, wpb, pgrid, s)
-              : launch_f32_mf<0>(a, lpf, bpl, n2_is_10, wpb, pgrid, s);
+    if (dmg) return launch_f32_mf<0, 1>(a, lpf, bpl, n2_is_10, wpb, pgrid, s);
+    return mf ? launch_f32_mf<1, 0>(a, lpf, bpl, n2_is_10, wpb, pgrid, s)
+              : launch_f32_mf<0, 0>(a, lpf, bpl, n2_is_10, wpb, pgrid, s);
 }
 
 }  // namespace pd

@@ -47,6 +47,9 @@ struct pd_context {
     double2* v0 = nullptr;
     double *V = nullptr, *E = nullptr, *kinv = nullptr;
     double* pf1 = nullptr;   // the uniform plane-stress factor (one value)
+    // Mazars bond damage
+    pd_damage_params dmgp = {PD_DAMAGE_OFF, 0, 0, 0, 0, 0};
+    double *kappa = nullptr, *dmg = nullptr, *dstat = nullptr;
     int *row = nullptr, *col = nullptr, *rev = nullptr, *bc = nullptr, *ids = nullptr;
     long ids_cap = 0;
     unsigned char* broken = nullptr;
@@ -142,7 +145,8 @@ static void free_all(pd_context* c)
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
     void* ptrs[] = {c->X, c->u, c->v, c->a, c->f, c->fown, c->v0, c->V, c->E, c->pf1, c->kinv,
-                    c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->W, c->counters};
+                    c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->W, c->counters,
+                    c->kappa, c->dmg, c->dstat};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -177,6 +181,15 @@ static BondArgs bond_args(pd_context* c)
     a.ktol = c->p.kappa_tol > 0.0 ? c->p.kappa_tol : 1e-10;
     a.force_k = 0;
     a.counters = c->counters;
+    a.dmg_mode = c->dmgp.mode;
+    a.kap0 = c->dmgp.kappa0;
+    a.At = c->dmgp.At;
+    a.Bt = c->dmgp.Bt;
+    a.Ac = c->dmgp.Ac;
+    a.Bc = c->dmgp.Bc;
+    a.kappa = c->kappa;
+    a.dmg = c->dmg;
+    a.dstat = c->dstat;
     return a;
 }
 
@@ -704,9 +717,76 @@ static int force_and_velocity(pd_context* c, int write_a)
     BondArgs a = bond_args(c);
     GatherArgs g = gather_args(c, write_a);
     int mixed = c->p.precision == PD_MIXED;
-    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->mf, c->n2_is_10, c->wpb, c->persist * c->n_cu, c->stream); }))
+    const int dmg = c->dmgp.mode != PD_DAMAGE_OFF;
+    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->mf, dmg, c->n2_is_10, c->wpb, c->persist * c->n_cu, c->stream); }))
         return rc;
     return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream); });
+}
+
+// ---------------------------------------------------------------------------
+// Mazars bond damage (computeDamageStatus, src/main_seq.cpp:473-598)
+// ---------------------------------------------------------------------------
+extern "C" int pd_set_damage(pd_context* c, const pd_damage_params* d)
+{
+    if (!c || !d) return fail(c, PD_ERR_INVALID, "NULL argument");
+    if (d->mode != PD_DAMAGE_OFF && d->mode != PD_DAMAGE_REFERENCE && d->mode != PD_DAMAGE_MAZARS)
+        return fail(c, PD_ERR_INVALID, "unknown damage mode %d", d->mode);
+    hipSetDevice(c->dev);
+    if (d->mode != PD_DAMAGE_OFF) {
+        if (!(d->kappa0 > 0) || !std::isfinite(d->At) || !std::isfinite(d->Bt) || !std::isfinite(d->Ac) ||
+            !std::isfinite(d->Bc))
+            return fail(c, PD_ERR_INVALID, "damage: kappa0 must be positive and A_t, B_t, A_c, B_c finite");
+        if (!c->kappa) {
+            if (int rc = dalloc(c, &c->kappa, c->nb)) return rc;
+            if (int rc = dalloc(c, &c->dmg, c->nb)) return rc;
+            if (int rc = dalloc(c, &c->dstat, c->n)) return rc;
+        }
+        std::vector<double> k0(std::max(c->nb, 1L), d->kappa0);
+        HIPCHK(c, hipMemcpy(c->kappa, k0.data(), sizeof(double) * k0.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(c->dmg, 0, sizeof(double) * std::max(c->nb, 1L)));
+        HIPCHK(c, hipMemset(c->dstat, 0, sizeof(double) * c->n));
+    }
+    c->dmgp = *d;
+    c->graph_valid = false;
+    return PD_OK;
+}
+
+extern "C" int pd_compute_damage(pd_context* c)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    if (c->dmgp.mode == PD_DAMAGE_OFF) return fail(c, PD_ERR_INVALID, "damage is off (pd_set_damage)");
+    hipSetDevice(c->dev);
+    BondArgs a = bond_args(c);
+    a.s0 = 0.0;  // no bond breaking in a damage-only evaluation
+    const int mixed = c->p.precision == PD_MIXED;
+    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->mf, 1, c->n2_is_10, c->wpb, 0, c->stream); }))
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PD_OK;
+}
+
+extern "C" int pd_get_damage(pd_context* c, double* bond_damage, double* status, double* kappa)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    if (!c->kappa) return fail(c, PD_ERR_INVALID, "damage was never enabled (pd_set_damage)");
+    hipSetDevice(c->dev);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> t(std::max(c->nb, (long)c->n));
+    if (bond_damage || kappa) {
+        for (int which = 0; which < 2; ++which) {
+            double* out = which ? kappa : bond_damage;
+            if (!out) continue;
+            if (c->nb)
+                HIPCHK(c, hipMemcpy(t.data(), which ? c->kappa : c->dmg, sizeof(double) * c->nb,
+                                    hipMemcpyDeviceToHost));
+            for (long b = 0; b < c->nb; ++b) out[c->bmap[b]] = t[b];
+        }
+    }
+    if (status) {
+        HIPCHK(c, hipMemcpy(t.data(), c->dstat, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+        for (int q = 0; q < c->n; ++q) status[c->perm[q]] = t[q];
+    }
+    return PD_OK;
 }
 
 extern "C" int pd_compute_velocity(pd_context* c, int write_acceleration)

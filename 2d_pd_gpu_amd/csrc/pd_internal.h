@@ -33,6 +33,12 @@ struct BondArgs {
     double ktol;              // singular-K tolerance
     int force_k;              // 1: assemble K for every family in this launch
     unsigned int* counters;   // [0] newly broken bonds, [1] singular K
+    // Mazars bond damage (kernels built with damage only)
+    int dmg_mode;             // 1: running threshold in the law (reference), 2: initial
+    double kap0, At, Bt, Ac, Bc;
+    double* kappa;            // per bond threshold
+    double* dmg;              // per bond damage
+    double* dstat;            // per particle damageStatus
 };
 
 struct GatherArgs {
@@ -66,11 +72,13 @@ struct PosArgs {
 bool tile_supported(int mixed, int lpf, int bpl, int mf);
 // pgrid > 0: persistent launch with pgrid workgroups (each walks a contiguous
 // range of family groups, prefetching the next group's loads).
-hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int mf, int n2_is_10, int wpb,
-                       int pgrid, hipStream_t s);
-hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int n2_is_10, int wpb,
+// dmg = 1: also update the Mazars bond damage (VALU contractions only; a
+// matrix-core tile runs its VALU twin).
+hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int mf, int dmg, int n2_is_10,
+                       int wpb, int pgrid, hipStream_t s);
+hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s);
-hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int n2_is_10, int wpb,
+hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s);
 hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int wpb, hipStream_t s);
 hipError_t launch_gather(const GatherArgs& a, int mixed, hipStream_t s);

@@ -121,12 +121,13 @@ bool tile_supported(int mixed, int lpf, int bpl, int mf)
     return false;
 }
 
-hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int mf, int n2_is_10, int wpb,
-                       int pgrid, hipStream_t s)
+hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int mf, int dmg, int n2_is_10,
+                       int wpb, int pgrid, hipStream_t s)
 {
     if (!tile_supported(mixed, lpf, bpl, mf)) return hipErrorInvalidValue;
-    return mixed ? launch_bond_f32(a, lpf, bpl, mf, n2_is_10, wpb, pgrid, s)
-                 : launch_bond_f64(a, lpf, bpl, mf, n2_is_10, wpb, pgrid, s);
+    if (dmg) mf = 0;
+    return mixed ? launch_bond_f32(a, lpf, bpl, mf, dmg, n2_is_10, wpb, pgrid, s)
+                 : launch_bond_f64(a, lpf, bpl, mf, dmg, n2_is_10, wpb, pgrid, s);
 }
 
 template <int LPF, int BPL, int N2>

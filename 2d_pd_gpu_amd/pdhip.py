@@ -37,6 +37,10 @@ OPT_XCD_REMAP = 4
 OPT_TILE = 5
 OPT_PERSISTENT = 6
 
+DAMAGE_OFF = 0
+DAMAGE_REFERENCE = 1
+DAMAGE_MAZARS = 2
+
 # every entry point include/pd_hip.h declares
 EXPORTS = (
     "pd_last_error", "pd_version", "pd_create", "pd_create_padded", "pd_destroy",
@@ -46,7 +50,23 @@ EXPORTS = (
     "pd_compute_velocity", "pd_set_boundary", "pd_step", "pd_autotune",
     "pd_get_broken", "pd_set_broken", "pd_get_bond_forces", "pd_get_stats",
     "pd_reset_profile", "pd_build_families", "pd_notch_mask", "pd_probe_copy",
+    "pd_set_damage", "pd_compute_damage", "pd_get_damage",
 )
+
+
+class DamageParams(C.Structure):
+    """pd_damage_params.  Defaults: the reference's Mazars constants
+    (src/main_seq.cpp:480) and threshold tensileStrength / E = 3 / 3.5e4
+    (:660, reference units)."""
+    _fields_ = [
+        ("mode", C.c_int), ("kappa0", C.c_double), ("At", C.c_double), ("Bt", C.c_double),
+        ("Ac", C.c_double), ("Bc", C.c_double),
+    ]
+
+    @classmethod
+    def make(cls, mode=DAMAGE_REFERENCE, kappa0=3.0 / 3.5e4, At=0.87, Bt=2.0e4, Ac=0.65,
+             Bc=2150.0):
+        return cls(mode, kappa0, At, Bt, Ac, Bc)
 
 
 class PDError(RuntimeError):
@@ -123,6 +143,9 @@ def lib():
         "pd_build_families": (C.c_long, [I, P, C.c_double, C.c_double, P, P, C.c_long]),
         "pd_notch_mask": (C.c_long, [I, P, P, P, I, P, P]),
         "pd_probe_copy": (I, [C.c_long, I, C.POINTER(C.c_double)]),
+        "pd_set_damage": (I, [ctx, C.POINTER(DamageParams)]),
+        "pd_compute_damage": (I, [ctx]),
+        "pd_get_damage": (I, [ctx, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -298,6 +321,25 @@ class Context:
         T = np.zeros(2 * max(self.nb, 1))
         self._check(lib().pd_get_bond_forces(self._h, _ptr(T)))
         return T[: 2 * self.nb]
+
+    # -- Mazars bond damage (computeDamageStatus) --
+    def set_damage(self, params=None, **kw):
+        """Enable (or, with mode=DAMAGE_OFF, stop) the damage update; resets
+        kappa = kappa0 and d = 0.  ``params``: DamageParams, or keywords of
+        DamageParams.make."""
+        dp = params if params is not None else DamageParams.make(**kw)
+        self._check(lib().pd_set_damage(self._h, C.byref(dp)))
+
+    def compute_damage(self):
+        self._check(lib().pd_compute_damage(self._h))
+
+    def get_damage(self):
+        """dict(bond=d per bond, status=damageStatus per particle, kappa=threshold per bond)."""
+        d = np.zeros(max(self.nb, 1))
+        k = np.zeros(max(self.nb, 1))
+        st = np.zeros(self.n)
+        self._check(lib().pd_get_damage(self._h, _ptr(d), _ptr(st), _ptr(k)))
+        return {"bond": d[: self.nb], "status": st, "kappa": k[: self.nb]}
 
     def stats(self):
         s = Stats()

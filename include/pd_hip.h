@@ -24,6 +24,8 @@
  *                            -> pd_create (CSR) / pd_create_padded (the GPU
  *                               path's padded layout, include/mechanics.cuh:
  *                               1033-1046) / pd_build_families
+ *   computeDamageStatus    src/main_seq.cpp:473-598, include/mechanics.h:24
+ *                            -> pd_set_damage / pd_compute_damage / pd_get_damage
  *   step loop              src/main_seq.cpp:680-715 -> pd_step
  * A header-only C++ adapter keeping the reference signatures is in
  * include/pd_mechanics_adapter.hpp.
@@ -68,6 +70,20 @@ extern "C" {
 #define PD_OPT_PERSISTENT    6  /* bond kernel as a persistent grid of value workgroups
                                    per CU that prefetch their next family group
                                    (0 = one wave per family group, the default)      */
+
+/* ---- Mazars bond damage (pd_set_damage) ------------------------------------ */
+#define PD_DAMAGE_OFF        0
+#define PD_DAMAGE_REFERENCE  1  /* the law as computeDamageStatus writes it: the
+                                   running threshold kappa_ik in d_t, d_c
+                                   (src/main_seq.cpp:579-582)                     */
+#define PD_DAMAGE_MAZARS     2  /* the published law: the initial threshold kappa0 */
+
+typedef struct pd_damage_params {
+    int    mode;      /* PD_DAMAGE_* */
+    double kappa0;    /* initial threshold (reference: tensileStrength / E, src/main_seq.cpp:660) */
+    double At, Bt;    /* tension law (reference: 0.87, 20000, src/main_seq.cpp:480) */
+    double Ac, Bc;    /* compression law (reference: 0.65, 2150) */
+} pd_damage_params;
 
 typedef struct pd_params {
     int    ndim;       /* must be 2 */
@@ -150,6 +166,28 @@ int pd_update_positions(pd_context* ctx);
  * != 0 also stores a = f/rho, as the reference GPU kernel does
  * (include/mechanics.cuh:863); the CPU path does not. */
 int pd_compute_velocity(pd_context* ctx, int write_acceleration);
+
+/* ---- Mazars bond damage -----------------------------------------------------
+ * computeDamageStatus (src/main_seq.cpp:473-598; include/mechanics.h:24):
+ * per bond, the principal strains of eps_ik = sym(F_ik) - I give
+ * eps_eq = sqrt(sum <l>+^2); past the bond's threshold kappa_ik the damage
+ * d_ik = alpha_t d_t + alpha_c d_c is recomputed and kappa_ik = eps_eq;
+ * damageStatus_i = sum_k d_ik / |H_i|.  A broken bond counts as d = 1.
+ * Damage does not feed back into the stress (as in the reference,
+ * src/main_seq.cpp:254).
+ * pd_set_damage: mode PD_DAMAGE_OFF stops the updates; any other mode resets
+ * kappa = kappa0 and d = 0.  While damage is on, every force evaluation
+ * (pd_compute_velocity, pd_step) also updates it at the positions it sees,
+ * fused into the bond kernel; pd_compute_damage updates it alone (no forces
+ * needed, no bond breaking).  An update at unchanged positions changes
+ * nothing, so the reference's order (damage at the start of a step, at the
+ * previous step's positions) gives the same state as the fused update one
+ * evaluation earlier. */
+int pd_set_damage(pd_context* ctx, const pd_damage_params* d);
+int pd_compute_damage(pd_context* ctx);
+/* Per-bond damage and threshold (caller CSR order, nb each) and per-particle
+ * damageStatus (caller order, n); NULL pointers are skipped. */
+int pd_get_damage(pd_context* ctx, double* bond_damage, double* status, double* kappa);
 
 /* ---- whole steps ------------------------------------------------------------ */
 /* Boundary program for pd_step: code[n] (caller order) with PD_BC_FREE,

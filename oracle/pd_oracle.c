@@ -148,10 +148,60 @@ static void plane_stress(double E, double nu, double e11, double e22, double e12
     *s12 = pf * (1.0 - nu) * e12;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Mazars damage, src/main_seq.cpp:473-598 (see pd_oracle.h).                */
+/* ------------------------------------------------------------------------ */
+void or_sym2_eig(double a, double b, double c, double* l1, double* l2)
+{
+    double m = 0.5 * (a + c);
+    double h = 0.5 * (a - c);
+    double r = sqrt(h * h + b * b);
+    *l1 = m - r;
+    *l2 = m + r;
+}
+
+int or_mazars_update(const or_damage_params* dp, double e11, double e22, double e12,
+                     double* kappa, double* d)
+{
+    double l[2];
+    or_sym2_eig(e11, e12, e22, &l[0], &l[1]);
+    double s2 = 0.0, eq = 0.0;
+    for (int i = 0; i < 2; ++i)
+        if (l[i] > 0.0) s2 += l[i] * l[i];
+    if (s2 > 0.0) eq = sqrt(s2);
+    if (!(eq > *kappa)) return 0;
+    /* eps_t and eps_c share eps's principal directions: their ascending
+     * spectra are (<l1>+, <l2>+) and (<l1>-, <l2>-). */
+    double t[2], c[2];
+    for (int i = 0; i < 2; ++i) {
+        t[i] = l[i] > 0.0 ? l[i] : 0.0;
+        c[i] = l[i] < 0.0 ? l[i] : 0.0;
+    }
+    double at = 0.0, ac = 0.0;
+    for (int i = 0; i < 2; ++i) {
+        if (t[i] > 0.0) at += t[i] * (t[i] + c[i]) / (eq * eq);
+        if (c[i] > 0.0) ac += c[i] * (t[i] + c[i]) / (eq * eq);
+    }
+    double kr = dp->mode == OR_DAMAGE_MAZARS ? dp->kappa0 : *kappa;
+    double dt = 1.0 - kr * (1.0 - dp->At) / eq - dp->At / exp(dp->Bt * (eq - kr));
+    double dc = 1.0 - kr * (1.0 - dp->Ac) / eq - dp->Ac / exp(dp->Bc * (eq - kr));
+    *d = at * dt + ac * dc;
+    *kappa = eq;
+    return 1;
+}
+
+typedef struct {
+    const or_damage_params* dp;
+    double* kappa;
+    double* dmg;
+    double* status;
+    long changed;
+} dmg_ws;
+
 static long family_T(const or_params* p, int i, const double* X, const double* V,
                      const double* E_arr, const int* row, const int* col,
                      const unsigned char* broken, const double* u, int bugcompat,
-                     double* T_out, double* Kinv_out, fam_ws* ws)
+                     double* T_out, double* Kinv_out, fam_ws* ws, dmg_ws* dw)
 {
     int beg = row[i];
     int m = row[i + 1] - beg;
@@ -187,7 +237,13 @@ static long family_T(const or_params* p, int i, const double* X, const double* V
             Kinv_out[3 * (size_t)(beg + k) + 1] = 0.0;
             Kinv_out[3 * (size_t)(beg + k) + 2] = 0.0;
         }
-        if (!ws->alive[k]) continue;
+        if (!ws->alive[k]) {
+            if (dw && dw->dmg[beg + k] != 1.0) {  /* broken: fully damaged */
+                dw->dmg[beg + k] = 1.0;
+                dw->changed++;
+            }
+            continue;
+        }
         VH += ws->Vl[k];
         double K00 = 0, K01 = 0, K11 = 0, D00 = 0, D01 = 0, D10 = 0, D11 = 0;
         for (int l = 0; l < m; ++l) {
@@ -229,6 +285,9 @@ static long family_T(const or_params* p, int i, const double* X, const double* V
             e22 = G11;
             e12 = 0.5 * (G01 + G10);
         }
+        if (dw)
+            dw->changed += or_mazars_update(dw->dp, e11, e22, e12, dw->kappa + beg + k,
+                                            dw->dmg + beg + k);
         double s11, s22, s12;
         plane_stress(E, p->nu, e11, e22, e12, &s11, &s22, &s12);
         double Vk = ws->Vl[k];
@@ -256,14 +315,50 @@ static long family_T(const or_params* p, int i, const double* X, const double* V
         T_out[2 * (size_t)(beg + jj)] = t0;
         T_out[2 * (size_t)(beg + jj) + 1] = t1;
     }
+    if (dw) {
+        double sum = 0.0;
+        for (int k = 0; k < m; ++k) sum += dw->dmg[beg + k];
+        dw->status[i] = m > 0 ? sum / m : 0.0;
+    }
     return nsing;
 }
+
+static long forces_impl(const or_params* p, int n, const double* X, const double* V,
+                        const double* E_arr, const int* row, const int* col,
+                        const int* rev, const unsigned char* broken, const double* u,
+                        int bugcompat, double* f_out, double* T_out, double* Kinv_out,
+                        const or_damage_params* dp, double* kappa, double* dmg,
+                        double* status, long* changed, int nthreads);
 
 long or_forces_fp64(const or_params* p, int n, const double* X, const double* V,
                     const double* E_arr, const int* row, const int* col,
                     const int* rev, const unsigned char* broken, const double* u,
                     int bugcompat, double* f_out, double* T_out, double* Kinv_out,
                     int nthreads)
+{
+    return forces_impl(p, n, X, V, E_arr, row, col, rev, broken, u, bugcompat, f_out, T_out,
+                       Kinv_out, NULL, NULL, NULL, NULL, NULL, nthreads);
+}
+
+long or_forces_damage_fp64(const or_params* p, const or_damage_params* dp, int n,
+                           const double* X, const double* V, const double* E_arr,
+                           const int* row, const int* col, const int* rev,
+                           const unsigned char* broken, const double* u, double* f_out,
+                           double* T_out, double* kappa, double* dmg, double* status,
+                           int nthreads)
+{
+    long changed = 0;
+    forces_impl(p, n, X, V, E_arr, row, col, rev, broken, u, 0, f_out, T_out, NULL, dp, kappa,
+                dmg, status, &changed, nthreads);
+    return changed;
+}
+
+static long forces_impl(const or_params* p, int n, const double* X, const double* V,
+                        const double* E_arr, const int* row, const int* col,
+                        const int* rev, const unsigned char* broken, const double* u,
+                        int bugcompat, double* f_out, double* T_out, double* Kinv_out,
+                        const or_damage_params* dp, double* kappa, double* dmg,
+                        double* status, long* changed, int nthreads)
 {
     long nb = row[n];
     double* T = T_out ? T_out : (double*)malloc(sizeof(double) * 2 * (size_t)(nb > 0 ? nb : 1));
@@ -272,21 +367,25 @@ long or_forces_fp64(const or_params* p, int n, const double* X, const double* V,
         rv = (int*)malloc(sizeof(int) * (size_t)(nb > 0 ? nb : 1));
         or_build_rev(n, row, col, rv);
     }
-    long nsing = 0;
+    long nsing = 0, nchg = 0;
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
-#pragma omp parallel num_threads(nthreads) reduction(+ : nsing)
+#pragma omp parallel num_threads(nthreads) reduction(+ : nsing, nchg)
 #endif
     {
         fam_ws ws;
         memset(&ws, 0, sizeof(ws));
+        dmg_ws dw = {dp, kappa, dmg, status, 0};
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 64)
 #endif
         for (int i = 0; i < n; ++i)
-            nsing += family_T(p, i, X, V, E_arr, row, col, broken, u, bugcompat, T, Kinv_out, &ws);
+            nsing += family_T(p, i, X, V, E_arr, row, col, broken, u, bugcompat, T, Kinv_out, &ws,
+                              dp ? &dw : NULL);
+        nchg += dw.changed;
         ws_free(&ws);
     }
+    if (changed) *changed = nchg;
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nthreads) schedule(static)
 #endif
@@ -313,6 +412,17 @@ long or_step_fp64(const or_params* p, int n, const double* X, const double* V,
                   const int* bc, const double* v0, double* u, double* v, double* a,
                   double* f, unsigned char* broken, int nsteps, int nthreads)
 {
+    return or_step_damage_fp64(p, NULL, n, X, V, E_arr, row, col, rev, bc, v0, u, v, a, f,
+                               broken, NULL, NULL, NULL, nsteps, nthreads);
+}
+
+long or_step_damage_fp64(const or_params* p, const or_damage_params* dp, int n,
+                         const double* X, const double* V, const double* E_arr,
+                         const int* row, const int* col, const int* rev, const int* bc,
+                         const double* v0, double* u, double* v, double* a, double* f,
+                         unsigned char* broken, double* kappa, double* dmg, double* status,
+                         int nsteps, int nthreads)
+{
     long nb = row[n];
     double* fn = (double*)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
     double* T = (double*)malloc(sizeof(double) * 2 * (size_t)(nb > 0 ? nb : 1));
@@ -337,7 +447,8 @@ long or_step_fp64(const or_params* p, int n, const double* X, const double* V,
             if (g == -2) { u[2 * i] = 0.0; u[2 * i + 1] = 0.0; }
         }
         if (broken) nbroken += or_break_bonds(p, n, X, row, col, u, broken);
-        or_forces_fp64(p, n, X, V, E_arr, row, col, rev, broken, u, 0, fn, T, NULL, nthreads);
+        forces_impl(p, n, X, V, E_arr, row, col, rev, broken, u, 0, fn, T, NULL, dp, kappa, dmg,
+                    status, NULL, nthreads);
         for (int i = 0; i < n; ++i) {
             for (int d = 0; d < 2; ++d) {
                 double an = fn[2 * i + d] / rho;

@@ -98,6 +98,64 @@ long or_step_fp64(const or_params* p, int n, const double* X, const double* V,
                   const int* bc, const double* v0, double* u, double* v, double* a,
                   double* f, unsigned char* broken, int nsteps, int nthreads);
 
+/* ---------------- Mazars bond damage (computeDamageStatus) ----------------- *
+ * Restatement of src/main_seq.cpp:473-598.  Per bond (i, k): eps = sym(F) - I
+ * (here sym(F - I) in displacement form), principal strains l1 <= l2 of the
+ * 2x2 symmetric eps (closed form: the reference calls Eigen's
+ * SelfAdjointEigenSolver, :523-526, Eigen 3.x, absent from this image),
+ * eps_eq = sqrt(sum <l>+^2) (:529-536).  When eps_eq > kappa_ik (:540):
+ *   tension part  eps_t = sum_{l>0} l v v^T, compression part eps_c =
+ *   sum_{l<0} l v v^T (:546-561; zero-initialised here: the reference leaves
+ *   strain_t / strain_c uninitialised, :509, :513), their ascending spectra
+ *   t_i, c_i (:564-568), alpha_t = sum_{t_i>0} t_i (t_i + c_i) / eps_eq^2,
+ *   alpha_c = sum_{c_i>0} c_i (t_i + c_i) / eps_eq^2 (:571-578),
+ *   d_t = 1 - kr (1 - A_t) / eps_eq - A_t / exp(B_t (eps_eq - kr)),
+ *   d_c likewise (:581-582), d = alpha_t d_t + alpha_c d_c, kappa_ik = eps_eq.
+ * kr is the running threshold kappa_ik in mode OR_DAMAGE_REFERENCE (what
+ * :579-582 compute) and the initial kappa0 in mode OR_DAMAGE_MAZARS (the
+ * published model).  damageStatus_i = sum_k d_ik / |H_i| (:593).
+ * Project rule for bond breaking (which the reference lacks): a broken bond
+ * has d = 1 and is not updated.
+ * Parity unpinned against the reference: no reference file holds damage
+ * values.  Pinned by closed forms in tests/test_oracle.py (principal strains
+ * of rotated tensors against numpy.linalg.eigvalsh, uniaxial / biaxial /
+ * mixed-sign eps_eq, idempotence at fixed strain, the affine patch test). */
+#define OR_DAMAGE_REFERENCE 1
+#define OR_DAMAGE_MAZARS    2
+
+typedef struct {
+    int mode;          /* OR_DAMAGE_REFERENCE or OR_DAMAGE_MAZARS */
+    double kappa0;     /* initial threshold (reference: tensileStrength / E, :660) */
+    double At, Bt, Ac, Bc;  /* :480: 0.87, 20000, 0.65, 2150 */
+} or_damage_params;
+
+/* Principal values l1 <= l2 of [[a, b], [b, c]]. */
+void or_sym2_eig(double a, double b, double c, double* l1, double* l2);
+
+/* One bond's update for strain (e11, e22, e12); returns 1 when eps_eq > kappa
+ * (kappa and d updated), else 0. */
+int or_mazars_update(const or_damage_params* dp, double e11, double e22, double e12,
+                     double* kappa, double* d);
+
+/* or_forces_fp64 plus the damage update of every bond at displacement u:
+ * kappa, dmg (nb each) updated in place, status (n) = sum_k d_ik / |H_i|.
+ * Returns the number of bonds whose damage changed. */
+long or_forces_damage_fp64(const or_params* p, const or_damage_params* dp, int n,
+                           const double* X, const double* V, const double* E_arr,
+                           const int* row, const int* col, const int* rev,
+                           const unsigned char* broken, const double* u, double* f_out,
+                           double* T_out, double* kappa, double* dmg, double* status,
+                           int nthreads);
+
+/* or_step_fp64 with the damage update after each step's force evaluation (at
+ * the same positions). */
+long or_step_damage_fp64(const or_params* p, const or_damage_params* dp, int n,
+                         const double* X, const double* V, const double* E_arr,
+                         const int* row, const int* col, const int* rev, const int* bc,
+                         const double* v0, double* u, double* v, double* a, double* f,
+                         unsigned char* broken, double* kappa, double* dmg, double* status,
+                         int nsteps, int nthreads);
+
 /* ---------------- literal long-double restatement of main_seq -------------- */
 
 /* Parameters exactly as the reference passes them: n1, n2, horizon, dx and

@@ -22,6 +22,23 @@ class OrParams(C.Structure):
     ]
 
 
+class DamageParams(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int), ("kappa0", C.c_double), ("At", C.c_double), ("Bt", C.c_double),
+        ("Ac", C.c_double), ("Bc", C.c_double),
+    ]
+
+
+DAMAGE_REFERENCE = 1
+DAMAGE_MAZARS = 2
+
+
+def damage_params(mode=DAMAGE_REFERENCE, kappa0=3.0 / 3.5e4, At=0.87, Bt=2.0e4, Ac=0.65,
+                  Bc=2150.0):
+    """Defaults are the reference's (src/main_seq.cpp:480, :660)."""
+    return DamageParams(mode, kappa0, At, Bt, Ac, Bc)
+
+
 class RefParams(C.Structure):
     _fields_ = [
         ("n1", C.c_double), ("n2", C.c_double), ("horizon", C.c_double),
@@ -54,6 +71,16 @@ def lib():
         L.or_break_bonds.argtypes = [C.POINTER(OrParams), C.c_int, P, P, P, P, P]
         L.or_step_fp64.restype = C.c_long
         L.or_step_fp64.argtypes = [C.POINTER(OrParams), C.c_int] + [P] * 13 + [C.c_int, C.c_int]
+        L.or_sym2_eig.restype = None
+        L.or_sym2_eig.argtypes = [C.c_double] * 3 + [P, P]
+        L.or_mazars_update.restype = C.c_int
+        L.or_mazars_update.argtypes = [C.POINTER(DamageParams)] + [C.c_double] * 3 + [P, P]
+        L.or_forces_damage_fp64.restype = C.c_long
+        L.or_forces_damage_fp64.argtypes = ([C.POINTER(OrParams), C.POINTER(DamageParams), C.c_int]
+                                            + [P] * 13 + [C.c_int])
+        L.or_step_damage_fp64.restype = C.c_long
+        L.or_step_damage_fp64.argtypes = ([C.POINTER(OrParams), C.POINTER(DamageParams), C.c_int]
+                                          + [P] * 16 + [C.c_int, C.c_int])
         L.ref_cpu_compute_velocity.restype = None
         L.ref_cpu_compute_velocity.argtypes = [C.POINTER(RefParams), C.c_int] + [P] * 8 + [C.c_int] * 3
         L.ref_cpu_update_positions.restype = None
@@ -126,6 +153,41 @@ def step(p, X, V, row, col, rev, bc, v0, u, v, a, f, broken, nsteps, E_arr=None,
     return lib().or_step_fp64(C.byref(p), n, _p(X), _p(V), _p(E_arr), _p(row), _p(col), _p(rev),
                               _p(bc), _p(v0), _p(u), _p(v), _p(a), _p(f), _p(broken), nsteps,
                               nthreads)
+
+
+def sym2_eig(a, b, c):
+    l1, l2 = C.c_double(), C.c_double()
+    lib().or_sym2_eig(float(a), float(b), float(c), C.byref(l1), C.byref(l2))
+    return l1.value, l2.value
+
+
+def mazars_update(dp, e11, e22, e12, kappa, d=0.0):
+    k, dd = C.c_double(kappa), C.c_double(d)
+    upd = lib().or_mazars_update(C.byref(dp), float(e11), float(e22), float(e12), C.byref(k),
+                                 C.byref(dd))
+    return bool(upd), k.value, dd.value
+
+
+def forces_damage(p, dp, X, V, row, col, u, kappa, dmg, broken=None, E_arr=None, rev=None,
+                  nthreads=8):
+    """Forces and the damage update at u; kappa, dmg updated in place.
+    Returns (f, status, changed)."""
+    n = len(V)
+    f = np.zeros(2 * n)
+    status = np.zeros(n)
+    changed = lib().or_forces_damage_fp64(C.byref(p), C.byref(dp), n, _p(X), _p(V), _p(E_arr),
+                                          _p(row), _p(col), _p(rev), _p(broken), _p(u), _p(f),
+                                          None, _p(kappa), _p(dmg), _p(status), nthreads)
+    return f, status, changed
+
+
+def step_damage(p, dp, X, V, row, col, rev, bc, v0, u, v, a, f, broken, kappa, dmg, status,
+                nsteps, E_arr=None, nthreads=8):
+    n = len(V)
+    return lib().or_step_damage_fp64(C.byref(p), C.byref(dp), n, _p(X), _p(V), _p(E_arr),
+                                     _p(row), _p(col), _p(rev), _p(bc), _p(v0), _p(u), _p(v),
+                                     _p(a), _p(f), _p(broken), _p(kappa), _p(dmg), _p(status),
+                                     nsteps, nthreads)
 
 
 # ---- literal long-double restatement -------------------------------------------------
