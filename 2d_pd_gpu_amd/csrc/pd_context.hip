@@ -765,6 +765,28 @@ extern "C" int pd_compute_damage(pd_context* c)
     return PD_OK;
 }
 
+extern "C" int pd_set_damage_state(pd_context* c, const double* bond_damage, const double* kappa)
+{
+    if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
+    if (c->dmgp.mode == PD_DAMAGE_OFF || !c->kappa)
+        return fail(c, PD_ERR_INVALID, "damage is off (pd_set_damage)");
+    hipSetDevice(c->dev);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> t(std::max(c->nb, 1L));
+    for (int which = 0; which < 2; ++which) {
+        const double* in = which ? kappa : bond_damage;
+        if (!in || !c->nb) continue;
+        for (long b = 0; b < c->nb; ++b) {
+            t[b] = in[c->bmap[b]];
+            if (!std::isfinite(t[b]))
+                return fail(c, PD_ERR_NOT_FINITE, "damage state: caller bond %ld is not finite", c->bmap[b]);
+        }
+        HIPCHK(c, hipMemcpy(which ? c->kappa : c->dmg, t.data(), sizeof(double) * c->nb,
+                            hipMemcpyHostToDevice));
+    }
+    return PD_OK;
+}
+
 extern "C" int pd_get_damage(pd_context* c, double* bond_damage, double* status, double* kappa)
 {
     if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");

@@ -50,7 +50,7 @@ EXPORTS = (
     "pd_compute_velocity", "pd_set_boundary", "pd_step", "pd_autotune",
     "pd_get_broken", "pd_set_broken", "pd_get_bond_forces", "pd_get_stats",
     "pd_reset_profile", "pd_build_families", "pd_notch_mask", "pd_probe_copy",
-    "pd_set_damage", "pd_compute_damage", "pd_get_damage",
+    "pd_set_damage", "pd_compute_damage", "pd_get_damage", "pd_set_damage_state",
 )
 
 
@@ -146,6 +146,7 @@ def lib():
         "pd_set_damage": (I, [ctx, C.POINTER(DamageParams)]),
         "pd_compute_damage": (I, [ctx]),
         "pd_get_damage": (I, [ctx, P, P, P]),
+        "pd_set_damage_state": (I, [ctx, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -332,6 +333,11 @@ class Context:
 
     def compute_damage(self):
         self._check(lib().pd_compute_damage(self._h))
+
+    def set_damage_state(self, bond_damage=None, kappa=None):
+        d = None if bond_damage is None else _f64(bond_damage, self.nb)
+        k = None if kappa is None else _f64(kappa, self.nb)
+        self._check(lib().pd_set_damage_state(self._h, _ptr(d), _ptr(k)))
 
     def get_damage(self):
         """dict(bond=d per bond, status=damageStatus per particle, kappa=threshold per bond)."""

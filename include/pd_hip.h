@@ -188,6 +188,10 @@ int pd_compute_damage(pd_context* ctx);
 /* Per-bond damage and threshold (caller CSR order, nb each) and per-particle
  * damageStatus (caller order, n); NULL pointers are skipped. */
 int pd_get_damage(pd_context* ctx, double* bond_damage, double* status, double* kappa);
+/* Load per-bond damage and threshold (caller CSR order; NULL skips), e.g. the
+ * caller-owned bondDamage / bondDamageThreshold of src/main_seq.cpp:643-676.
+ * Damage must be on. */
+int pd_set_damage_state(pd_context* ctx, const double* bond_damage, const double* kappa);
 
 /* ---- whole steps ------------------------------------------------------------ */
 /* Boundary program for pd_step: code[n] (caller order) with PD_BC_FREE,

@@ -4,6 +4,8 @@
 // A driver written against EricYJA/2D_PD_GPU's
 //   compute_velocity_GPU_host(...)   include/mechanics.cuh:867-881 (called at src/main.cu:628)
 //   computeVelocity(...)             src/main_seq.cpp:396-398       (called at src/main_seq.cpp:698)
+//   computeDamageStatus(...)         src/main_seq.cpp:473, include/mechanics.h:24
+//                                    (called at src/main_seq.cpp:683)
 // recompiles unchanged against this header: the functions are templates over the
 // driver's own `Particle` (include/SetupParticleSystem.h:11-22) and `matrix`
 // (include/matrix.h:7-26) types, so no reference header is needed here.
@@ -70,7 +72,59 @@ uint64_t mesh_signature(const std::vector<ParticleT>& ps)
     return h;
 }
 
-// Shared body of both entry points.
+// The cached context for this mesh; created (or re-created) when the mesh or
+// the parameters change.  geometry_only: any material will do (damage needs
+// only the families and the weights), so an existing context for the same mesh
+// and weight parameters is kept.
+template <class ParticleT>
+pd_context* mesh_context(const pd_params& p, const std::vector<ParticleT>& allParticles,
+                         const std::unordered_map<int, int>& globalLocalIDmap, bool geometry_only)
+{
+    const int n = (int)allParticles.size();
+    Cache& c = cache();
+    const uint64_t sig = mesh_signature(allParticles);
+    const bool same_mesh = c.ctx && c.n == n && c.sig == sig && c.p.n1 == p.n1 && c.p.n2 == p.n2 &&
+                           c.p.horizon == p.horizon && c.p.dx == p.dx;
+    const bool same = same_mesh && (geometry_only || (c.p.E == p.E && c.p.nu == p.nu &&
+                                                      c.p.rho == p.rho && c.p.dt == p.dt));
+    if (!same) {
+        if (c.ctx) pd_destroy(c.ctx);
+        c.ctx = nullptr;
+        std::vector<double> X(2 * (size_t)n), V(n);
+        std::vector<int> row(n + 1, 0), col;
+        for (int i = 0; i < n; ++i) {
+            const ParticleT& q = allParticles[i];
+            X[2 * i] = (double)q.initialPositions[0];
+            X[2 * i + 1] = (double)q.initialPositions[1];
+            V[i] = (double)q.volume;
+            for (int g : q.neighbors) {
+                auto it = globalLocalIDmap.find(g);
+                if (it != globalLocalIDmap.end()) col.push_back(it->second);
+            }
+            row[i + 1] = (int)col.size();
+        }
+        check(pd_create(&p, n, X.data(), V.data(), row.data(), col.data(), &c.ctx), nullptr);
+        c.n = n;
+        c.sig = sig;
+        c.p = p;
+    }
+    return c.ctx;
+}
+
+// u = y - X, formed in long double before rounding.
+template <class ParticleT>
+std::vector<double> displacement(const std::vector<ParticleT>& allParticles)
+{
+    const int n = (int)allParticles.size();
+    std::vector<double> u(2 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 2; ++d)
+            u[2 * i + d] = (double)(allParticles[i].currentPositions[d] -
+                                    allParticles[i].initialPositions[d]);
+    return u;
+}
+
+// Shared body of both velocity entry points.
 template <class ParticleT, class MatrixT>
 void compute_velocity(double n1, double n2, double horizon, double dx, double massDensity,
                       MatrixT& StiffnessTensor, double stepSize,
@@ -98,50 +152,83 @@ void compute_velocity(double n1, double n2, double horizon, double dx, double ma
     p.s0 = 0.0;  // the reference never breaks bonds
     p.kappa_tol = 0.0;
 
-    Cache& c = cache();
-    const uint64_t sig = mesh_signature(allParticles);
-    const bool same = c.ctx && c.n == n && c.sig == sig && c.p.E == p.E && c.p.nu == p.nu &&
-                      c.p.n1 == p.n1 && c.p.n2 == p.n2 && c.p.horizon == p.horizon &&
-                      c.p.dx == p.dx && c.p.rho == p.rho && c.p.dt == p.dt;
-    if (!same) {
-        if (c.ctx) pd_destroy(c.ctx);
-        c.ctx = nullptr;
-        std::vector<double> X(2 * (size_t)n), V(n);
-        std::vector<int> row(n + 1, 0), col;
-        for (int i = 0; i < n; ++i) {
-            const ParticleT& q = allParticles[i];
-            X[2 * i] = (double)q.initialPositions[0];
-            X[2 * i + 1] = (double)q.initialPositions[1];
-            V[i] = (double)q.volume;
-            for (int g : q.neighbors) {
-                auto it = globalLocalIDmap.find(g);
-                if (it != globalLocalIDmap.end()) col.push_back(it->second);
-            }
-            row[i + 1] = (int)col.size();
-        }
-        check(pd_create(&p, n, X.data(), V.data(), row.data(), col.data(), &c.ctx), nullptr);
-        c.n = n;
-        c.sig = sig;
-        c.p = p;
-    }
-    std::vector<double> u(2 * (size_t)n), v(2 * (size_t)n), a(2 * (size_t)n), f(2 * (size_t)n);
+    pd_context* ctx = mesh_context(p, allParticles, globalLocalIDmap, false);
+    std::vector<double> u = displacement(allParticles);
+    std::vector<double> v(2 * (size_t)n), a(2 * (size_t)n), f(2 * (size_t)n);
     for (int i = 0; i < n; ++i)
         for (int d = 0; d < 2; ++d) {
-            const ParticleT& q = allParticles[i];
-            u[2 * i + d] = (double)(q.currentPositions[d] - q.initialPositions[d]);
             v[2 * i + d] = (double)velocity[i][d];
             a[2 * i + d] = (double)acce[i][d];
             f[2 * i + d] = (double)netF[i][d];
         }
-    check(pd_set_state(c.ctx, u.data(), v.data(), a.data(), f.data()), c.ctx);
-    check(pd_compute_velocity(c.ctx, write_acce ? 1 : 0), c.ctx);
-    check(pd_get_state(c.ctx, nullptr, v.data(), a.data(), f.data()), c.ctx);
+    check(pd_set_state(ctx, u.data(), v.data(), a.data(), f.data()), ctx);
+    check(pd_compute_velocity(ctx, write_acce ? 1 : 0), ctx);
+    check(pd_get_state(ctx, nullptr, v.data(), a.data(), f.data()), ctx);
     for (int i = 0; i < n; ++i)
         for (int d = 0; d < 2; ++d) {
             velocity[i][d] = (long double)v[2 * i + d];
             acce[i][d] = write_acce ? (long double)a[2 * i + d] : (long double)(double)acce[i][d];
             netF[i][d] = (long double)f[2 * i + d];
         }
+}
+
+// computeDamageStatus body: the caller owns the per-bond damage and thresholds
+// (bondDamage / bondDamageThreshold, indexed [particle][position in its
+// neighbour list]); they are loaded, updated at the particles' current
+// positions with the reference's Mazars constants (src/main_seq.cpp:480), and
+// written back, with Particle::damageStatus.
+template <class ParticleT>
+void compute_damage(double n1, double n2, double dx, double horizon,
+                    std::vector<std::vector<double>>& bondDamageThreshold,
+                    std::vector<ParticleT>& localParticles,
+                    std::vector<std::vector<double>>& bondDamage,
+                    const std::unordered_map<int, int>& globalLocalIDmap)
+{
+    const int n = (int)localParticles.size();
+    pd_params p{};
+    p.ndim = 2;
+    p.precision = PD_FP64;
+    p.n1 = n1;
+    p.n2 = n2;
+    p.horizon = horizon;
+    p.dx = dx;
+    p.rho = p.dt = p.E = 1.0;  // not used by the damage update
+    p.nu = 0.0;
+    pd_context* ctx = mesh_context(p, localParticles, globalLocalIDmap, true);
+    // the bond order of the context: neighbours found in the id map, in order
+    std::vector<double> d, k;
+    std::vector<int> cnt(n, 0);
+    for (int i = 0; i < n; ++i) {
+        int j = 0;
+        for (int g : localParticles[i].neighbors) {
+            if (globalLocalIDmap.find(g) == globalLocalIDmap.end()) continue;
+            d.push_back(bondDamage.at(i).at(j));
+            k.push_back(bondDamageThreshold.at(i).at(j));
+            ++j;
+        }
+        cnt[i] = j;
+    }
+    pd_damage_params dp{PD_DAMAGE_REFERENCE, 1.0, 0.87, 20000.0, 0.65, 2150.0};
+    check(pd_set_damage(ctx, &dp), ctx);
+    check(pd_set_damage_state(ctx, d.data(), k.data()), ctx);
+    std::vector<double> u = displacement(localParticles);
+    check(pd_set_state(ctx, u.data(), nullptr, nullptr, nullptr), ctx);
+    check(pd_compute_damage(ctx), ctx);
+    std::vector<double> st(n);
+    check(pd_get_damage(ctx, d.data(), st.data(), k.data()), ctx);
+    // the caller owns the state: stop updating it in later force evaluations
+    dp.mode = PD_DAMAGE_OFF;
+    check(pd_set_damage(ctx, &dp), ctx);
+    size_t e = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < cnt[i]; ++j, ++e) {
+            bondDamage[i][j] = d[e];
+            bondDamageThreshold[i][j] = k[e];
+        }
+        // damageStatus divides by the neighbour count (src/main_seq.cpp:593)
+        const size_t m = localParticles[i].neighbors.size();
+        localParticles[i].damageStatus = m ? st[i] * cnt[i] / (double)m : 0.0;
+    }
 }
 
 }  // namespace pd_adapter
@@ -183,4 +270,19 @@ void computeVelocity(int ndim, double n1, double n2, double horizon, double dx,
     pd_adapter::compute_velocity(n1, n2, horizon, dx, (double)massDensity, StiffnessTensor,
                                  (double)stepSize, velocity, acce, netF, localParticles,
                                  globalLocalIDmap, false);
+}
+
+// src/main_seq.cpp:473 (include/mechanics.h:24)
+template <class ParticleT>
+void computeDamageStatus(int ndim, double n1, double n2, double dx, double horizon,
+                         std::vector<std::vector<double>>& bondDamageThreshold,
+                         std::vector<ParticleT>& localParticles,
+                         const std::vector<std::vector<ParticleT*>>& Neighborslist,
+                         std::vector<std::vector<double>>& bondDamage,
+                         const std::unordered_map<int, int>& globalLocalIDmap)
+{
+    (void)Neighborslist;
+    if (ndim != 2) throw std::runtime_error("lib2dpdgpu: only ndim == 2 is supported");
+    pd_adapter::compute_damage(n1, n2, dx, horizon, bondDamageThreshold, localParticles, bondDamage,
+                               globalLocalIDmap);
 }

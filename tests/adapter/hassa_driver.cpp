@@ -4,8 +4,14 @@
 // Its Particle and matrix types are written here with the members the
 // reference's types expose (include/SetupParticleSystem.h:11-22,
 // include/matrix.h:7-26), to show that a driver using them links unchanged.
-// Usage: hassa_driver <mesh.dat> <out.txt> [gpu|seq]
-// Writes "vx vy" per particle with 17 significant digits.
+// Usage: hassa_driver <mesh.dat> <out.txt> [gpu|seq|damage]
+// gpu / seq: one step through compute_velocity_GPU_host / computeVelocity;
+// writes "vx vy" per particle with 17 significant digits.
+// damage: computeDamageStatus (src/main_seq.cpp:473) twice at the prescribed
+// displacement u_x = s (0.7 x + 0.2 y + 0.01 y^2), u_y = s (0.4 y - 0.1 x),
+// s = 4e-4, with the reference's initial state (bondDamage = 0,
+// bondDamageThreshold = 3 / 3.5e4, :660-672); writes "status" per particle to
+// out.txt and "d kappa" per bond (neighbour-list order) to out.txt.bonds.
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -50,6 +56,7 @@ int main(int argc, char** argv)
         return 2;
     }
     const bool gpu_entry = argc < 4 || string(argv[3]) == "gpu";
+    const bool damage_entry = argc >= 4 && string(argv[3]) == "damage";
     // mesh with two edge notches removed (readMeshFile semantics)
     ifstream in(argv[1]);
     string line;
@@ -94,6 +101,34 @@ int main(int argc, char** argv)
     vector<vector<long double>> netF(n, vector<long double>(2, 0.0)), velocity = netF, acce = netF;
     vector<vector<Particle*>> nl(n);
     vector<vector<double>> bondDamage(n + 1);
+    if (damage_entry) {
+        vector<vector<double>> threshold(n);
+        for (int i = 0; i < n; ++i) {
+            bondDamage[i].assign(ps[i].neighbors.size(), 0.0);
+            threshold[i].assign(ps[i].neighbors.size(), 3.0 / 3.5e4);
+            const long double x = ps[i].initialPositions[0], y = ps[i].initialPositions[1];
+            const long double s = 4e-4L;
+            ps[i].currentPositions[0] = x + s * (0.7L * x + 0.2L * y + 0.01L * y * y);
+            ps[i].currentPositions[1] = y + s * (0.4L * y - 0.1L * x);
+        }
+        try {
+            for (int rep = 0; rep < 2; ++rep)
+                computeDamageStatus(2, 10.0, 10.0, 1.0, 3.0, threshold, ps, nl, bondDamage, id);
+        } catch (const exception& e) {
+            cerr << e.what() << "\n";
+            return 1;
+        }
+        ofstream out(argv[2]), ob(string(argv[2]) + ".bonds");
+        out << setprecision(17);
+        ob << setprecision(17);
+        for (int i = 0; i < n; ++i) {
+            out << ps[i].damageStatus << "\n";
+            for (size_t j = 0; j < ps[i].neighbors.size(); ++j)
+                ob << bondDamage[i][j] << " " << threshold[i][j] << "\n";
+        }
+        cout << "particles " << n << "\n";
+        return 0;
+    }
     const long double stepSize = 1.0e-9L, rho = 2.5e-6L, E = 3.5e4L, nv = 0.2L;
     double pf = (double)E / (1 - pow((double)nv, 2));
     matrix C(3, {1.0, (double)nv, 0.0, (double)nv, 1.0, 0.0, 0.0, 0.0, 1 - (double)nv});

@@ -135,3 +135,36 @@ def test_damage_persistent_and_off(pdpkg, plate):
     with pytest.raises(pdh.PDError):
         ctx.compute_damage()  # damage never enabled
     ctx.close()
+
+
+def test_adapter_compute_damage_status(tmp_path):
+    """A driver calling computeDamageStatus with the reference signature
+    (include/pd_mechanics_adapter.hpp) against the oracle, reference mode and
+    constants, at the driver's prescribed displacement."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    say("adapter computeDamageStatus (reference signature)")
+    X, V, row, col, sets = H.case()
+    X = X.astype(np.float64)
+    x, y = X[0::2], X[1::2]
+    s = 4e-4
+    u = np.empty_like(X)
+    u[0::2] = s * (0.7 * x + 0.2 * y + 0.01 * y * y)
+    u[1::2] = s * (0.4 * y - 0.1 * x)
+    kap0 = 3.0 / 3.5e4
+    kap = np.full(len(col), kap0)
+    dm = np.zeros(len(col))
+    p = O.or_params(horizon=H.HORIZON, dx=H.DX, rho=H.RHO, dt=H.DT, E=H.E, nu=H.NU)
+    _, st, changed = O.forces_damage(p, O.damage_params(kappa0=kap0), X, V.astype(np.float64), row,
+                                     col, u, kap, dm)
+    assert changed > len(col) // 4
+    exe = os.path.join(ROOT, "tests", "adapter", "build", "hassa_driver")
+    out = tmp_path / "dmg.txt"
+    r = subprocess.run([exe, H.MESH, str(out), "damage"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    status = np.loadtxt(out)
+    bonds = np.loadtxt(str(out) + ".bonds")
+    assert np.linalg.norm(bonds[:, 1] - kap) <= 1e-10 * np.linalg.norm(kap)
+    assert np.linalg.norm(bonds[:, 0] - dm) <= 1e-9 * np.linalg.norm(dm)
+    assert np.linalg.norm(status - st) <= 1e-9 * np.linalg.norm(st)
