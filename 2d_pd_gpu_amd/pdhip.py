@@ -51,6 +51,7 @@ EXPORTS = (
     "pd_get_broken", "pd_set_broken", "pd_get_bond_forces", "pd_get_stats",
     "pd_reset_profile", "pd_build_families", "pd_notch_mask", "pd_probe_copy",
     "pd_set_damage", "pd_compute_damage", "pd_get_damage", "pd_set_damage_state",
+    "pd_read_mesh", "pd_write_velocity", "pd_write_positions", "pd_write_gathered",
 )
 
 
@@ -147,6 +148,10 @@ def lib():
         "pd_compute_damage": (I, [ctx]),
         "pd_get_damage": (I, [ctx, P, P, P]),
         "pd_set_damage_state": (I, [ctx, P, P]),
+        "pd_read_mesh": (C.c_long, [C.c_char_p, I, P, P, C.c_long]),
+        "pd_write_velocity": (I, [ctx, C.c_char_p, I]),
+        "pd_write_positions": (I, [ctx, C.c_char_p, P, I]),
+        "pd_write_gathered": (I, [ctx, C.c_char_p, P, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -185,6 +190,19 @@ def build_families(X, dx, horizon):
     if nb2 != nb:
         raise PDError(PD_ERR_INVALID, "pd_build_families failed")
     return row, col[:nb]
+
+
+def read_mesh(path, reference_notch=True):
+    """readMeshFile (src/SetupParticleSystem.cpp:11-60): returns X (2n), V (n)."""
+    p = os.fsencode(path)
+    n = lib().pd_read_mesh(p, int(bool(reference_notch)), None, None, 0)
+    if n < 0:
+        raise PDError(PD_ERR_INVALID, lib().pd_last_error(None).decode())
+    X = np.zeros(2 * max(n, 1))
+    V = np.zeros(max(n, 1))
+    if lib().pd_read_mesh(p, int(bool(reference_notch)), _ptr(X), _ptr(V), n) != n:
+        raise PDError(PD_ERR_INVALID, lib().pd_last_error(None).decode())
+    return X[: 2 * n], V[:n]
 
 
 def probe_copy(nbytes=1 << 30, iters=20):
@@ -346,6 +364,18 @@ class Context:
         st = np.zeros(self.n)
         self._check(lib().pd_get_damage(self._h, _ptr(d), _ptr(st), _ptr(k)))
         return {"bond": d[: self.nb], "status": st, "kappa": k[: self.nb]}
+
+    # -- result files in the reference's formats --
+    def write_velocity(self, path, digits=0):
+        self._check(lib().pd_write_velocity(self._h, os.fsencode(path), int(digits)))
+
+    def write_positions(self, path, ids=None, digits=0):
+        ids = None if ids is None else _i32(ids)
+        self._check(lib().pd_write_positions(self._h, os.fsencode(path), _ptr(ids), int(digits)))
+
+    def write_gathered(self, path, ids=None, digits=0):
+        ids = None if ids is None else _i32(ids)
+        self._check(lib().pd_write_gathered(self._h, os.fsencode(path), _ptr(ids), int(digits)))
 
     def stats(self):
         s = Stats()

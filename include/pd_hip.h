@@ -27,6 +27,9 @@
  *   computeDamageStatus    src/main_seq.cpp:473-598, include/mechanics.h:24
  *                            -> pd_set_damage / pd_compute_damage / pd_get_damage
  *   step loop              src/main_seq.cpp:680-715 -> pd_step
+ *   readMeshFile, storeVelocity, outputLocalParticlesPositions,
+ *   outputGatheredPositions -> pd_read_mesh, pd_write_velocity,
+ *                              pd_write_positions, pd_write_gathered
  * A header-only C++ adapter keeping the reference signatures is in
  * include/pd_mechanics_adapter.hpp.
  */
@@ -213,6 +216,25 @@ int pd_set_broken(pd_context* ctx, const unsigned char* mask);
 int pd_get_bond_forces(pd_context* ctx, double* T);
 int pd_get_stats(pd_context* ctx, pd_stats* out);
 void pd_reset_profile(pd_context* ctx);
+
+/* ---- mesh and result files in the reference's formats ------------------------
+ * Text through a C++ stream as the reference writes it; digits <= 0 keeps the
+ * stream default (6 significant digits, the reference's), 17 reproduces every
+ * double.  ids: the particles' global ids in caller order (NULL: 0..n-1). */
+/* readMeshFile (src/SetupParticleSystem.cpp:11-60): a header line, then
+ * "x y area" per line.  reference_notch != 0 drops |x| > 17.5 && |y| < 2
+ * (:33), the reference's two edge notches.  X (2*cap) and V (cap) may both be
+ * NULL to count.  Returns the particle count, or -1 (pd_last_error(NULL)). */
+long pd_read_mesh(const char* path, int reference_notch, double* X, double* V, long cap);
+/* storeVelocity (include/util.h:20-39): "vx vy" per particle. */
+int pd_write_velocity(pd_context* ctx, const char* path, int digits);
+/* outputLocalParticlesPositions (src/SetupParticleSystem.cpp:646-658):
+ * header "ID, x, y, ux, uy ", then "id,x,y,ux,uy" (initial x, y; u = y - X). */
+int pd_write_positions(pd_context* ctx, const char* path, const int* ids, int digits);
+/* outputGatheredPositions (src/SetupParticleSystem.cpp:552-627), the rank-0
+ * file of a one-GPU run: "ID, x, y, ux, uy, damage" with damageStatus (0 when
+ * damage was never enabled). */
+int pd_write_gathered(pd_context* ctx, const char* path, const int* ids, int digits);
 
 /* ---- diagnostics (no counterpart in the reference) ---------------------------
  * Times a plain device-to-device copy kernel that reads and writes `bytes`

@@ -137,3 +137,20 @@ def test_workload_counts_match_unjittered_counts(pdpkg):
     (272,836): the 1.01 cutoff factor absorbs the +-0.01 dx jitter."""
     w = pdpkg.plates.plate_100()
     assert w.n == 10000 and w.nb == 272836
+
+
+def test_read_mesh_matches_reference_reader(pdpkg, tmp_path):
+    """pd_read_mesh keeps readMeshFile's rules (src/SetupParticleSystem.cpp:11-60):
+    header skipped, notch particles |x| > 17.5 and |y| < 2 dropped."""
+    pdh = pdpkg.pdhip
+    X, V = pdh.read_mesh(H.MESH)
+    Xr, Vr = H.read_mesh()
+    assert V.size == 4064
+    np.testing.assert_array_equal(X, Xr.astype(np.float64))
+    np.testing.assert_array_equal(V, Vr.astype(np.float64))
+    Xa, Va = pdh.read_mesh(H.MESH, reference_notch=False)
+    with open(H.MESH) as fh:
+        assert Va.size == sum(1 for line in fh.readlines()[1:] if len(line.split()) >= 3)
+    bad = tmp_path / "missing.dat"
+    with pytest.raises(pdh.PDError):
+        pdh.read_mesh(str(bad))
