@@ -584,7 +584,12 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
     const int lk = lane & 15, lq = lane >> 4, lc = lane & 3;
     double wc[WF][NQ][NS];   // FP64 matrix-core weight cache
     int Mf[WF];              // family sizes (wave-uniform)
-    R w[MF64 ? 1 : BPL][MF64 ? 1 : FS];  // per-lane weight rows otherwise
+    // Families of 64 slots evaluate the weights again in contraction 2 rather
+    // than keep a row of 64 weights live per lane: the occupancy gained
+    // outweighs the extra VALU work there (measured: FP64 -24 %, mixed -4 %),
+    // not for 32-slot families (+7 % and more).
+    constexpr bool RW = !MF && FS >= 64;
+    R w[(MF64 || RW) ? 1 : BPL][(MF64 || RW) ? 1 : FS];  // per-lane weight rows otherwise
     R D[BPL][4];
     v4f Dv[BPL];
 #pragma unroll
@@ -662,7 +667,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
-                        w[s][l] = om;
+                        if constexpr (!RW) w[s][l] = om;
                         Dv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Pa, om, Dv[s], 0, 0, 0);
                     }
                 } else if constexpr (COMPACT) {
@@ -671,7 +676,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
-                        w[s][l] = om;
+                        if constexpr (!RW) w[s][l] = om;
                         const R t0 = om * a0, t1 = om * a1;
                         D[s][0] += t0 * ex;
                         D[s][1] += t0 * ey;
@@ -686,7 +691,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
-                        w[s][l] = om;
+                        if constexpr (!RW) w[s][l] = om;
                         D[s][0] += om * P0;
                         D[s][1] += om * P1;
                         D[s][2] += om * P2;
@@ -698,7 +703,8 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
             for (int l = l0; l < l0 + 4; ++l)
 #pragma unroll
-                for (int s = 0; s < BPL; ++s) w[s][l] = R(0);
+                for (int s = 0; s < BPL; ++s)
+                    if constexpr (!RW) w[s][l] = R(0);
         }
         // keep the scheduler from hoisting every record load of the unrolled
         // loop (which would spill)
@@ -818,12 +824,24 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
                     const R Q1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
                     const R Q2 = rec[rec_at<R, FPW>(l, F.slot, 6)];
                     const R Q3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
+                    R ex = R(0), ey = R(0), pe = R(0), qe = R(0);
+                    if constexpr (RW) {
+                        ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
+                        ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
+                        pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
+                        qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
+                    }
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
-                        Sm[s][0] += w[s][l] * Q0;
-                        Sm[s][1] += w[s][l] * Q1;
-                        Sm[s][2] += w[s][l] * Q2;
-                        Sm[s][3] += w[s][l] * Q3;
+                        R om;
+                        if constexpr (RW)
+                            om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        else
+                            om = w[s][l];
+                        Sm[s][0] += om * Q0;
+                        Sm[s][1] += om * Q1;
+                        Sm[s][2] += om * Q2;
+                        Sm[s][3] += om * Q3;
                     }
                 }
             }
