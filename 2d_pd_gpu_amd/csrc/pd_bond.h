@@ -589,6 +589,10 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
     // outweighs the extra VALU work there (measured: FP64 -24 %, mixed -4 %),
     // not for 32-slot families (+7 % and more).
     constexpr bool RW = !MF && FS >= 64;
+    // neighbours per scheduling group of the contraction loops (each group's
+    // record loads are issued together; 2 frees no registers in the FP64
+    // two-bonds-per-lane tile, whose live set is its two weight rows)
+    constexpr int GL = 4;
     R w[(MF64 || RW) ? 1 : BPL][(MF64 || RW) ? 1 : FS];  // per-lane weight rows otherwise
     R D[BPL][4];
     v4f Dv[BPL];
@@ -654,10 +658,10 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
             for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
     } else {
 #pragma unroll
-    for (int l0 = 0; l0 < FS; l0 += 4) {
+    for (int l0 = 0; l0 < FS; l0 += GL) {
         if (l0 < mmax) {
 #pragma unroll
-            for (int l = l0; l < l0 + 4; ++l) {
+            for (int l = l0; l < l0 + GL; ++l) {
                 const R ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
                 const R ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
                 const R pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
@@ -701,7 +705,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
             }
         } else {
 #pragma unroll
-            for (int l = l0; l < l0 + 4; ++l)
+            for (int l = l0; l < l0 + GL; ++l)
 #pragma unroll
                 for (int s = 0; s < BPL; ++s)
                     if constexpr (!RW) w[s][l] = R(0);
@@ -709,7 +713,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
         // keep the scheduler from hoisting every record load of the unrolled
         // loop (which would spill)
         __builtin_amdgcn_sched_barrier(0);
-        if (l0 == FS / 2 - 4) hook_a();
+        if (l0 == FS / 2 - GL) hook_a();
     }
     hook_b();
     if constexpr (MF32) {
@@ -810,10 +814,10 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
     for (int s = 0; s < BPL; ++s) Sv[s] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int l0 = 0; l0 < FS; l0 += 4) {
+    for (int l0 = 0; l0 < FS; l0 += GL) {
         if (l0 < mmax) {
 #pragma unroll
-            for (int l = l0; l < l0 + 4; ++l) {
+            for (int l = l0; l < l0 + GL; ++l) {
                 if constexpr (MF32) {
                     const R Qa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
 #pragma unroll
