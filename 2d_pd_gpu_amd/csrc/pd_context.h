@@ -131,3 +131,10 @@ int dalloc(pd_context* c, T** p, size_t count)
                     hipGetErrorString(e));
     return PD_OK;
 }
+
+// Mesh arrays in the internal order (X, V, row, col, rev) for c->n, c->nb.
+int alloc_mesh(pd_context* c);
+// pd_create_from_points: families, Morton order, internal CSR and reverse
+// bonds built on the device (pd_setup.hip); sets c->nb, c->maxM and the host
+// mirrors.
+int build_mesh_on_device(pd_context* c, const double* X, const double* V);

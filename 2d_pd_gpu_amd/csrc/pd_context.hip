@@ -202,6 +202,77 @@ static int check_params(const pd_params* p)
     return PD_OK;
 }
 
+// Device, CU count, the context's own stream and timing events.
+static int device_open(pd_context* c)
+{
+    if (hipGetDevice(&c->dev) != hipSuccess) return fail(c, PD_ERR_HIP, "no HIP device available");
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
+        c->n_cu <= 0)
+        c->n_cu = 256;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(c, PD_ERR_HIP, "hipStreamCreate failed");
+    c->own_stream = true;
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return fail(c, PD_ERR_HIP, "hipEventCreate failed");
+    return PD_OK;
+}
+
+// Mesh arrays in the internal order: X, V (n), row (n + 1), col, rev (nb).
+int alloc_mesh(pd_context* c)
+{
+    int rc = PD_OK;
+    (void)((rc = dalloc(c, &c->X, c->n)) || (rc = dalloc(c, &c->V, c->n)) ||
+           (rc = dalloc(c, &c->row, (size_t)c->n + 1)) || (rc = dalloc(c, &c->col, c->nb)) ||
+           (rc = dalloc(c, &c->rev, c->nb)));
+    return rc;
+}
+
+// State and per-bond buffers, their initial values, and K^-1 of every bond.
+static int finish_create(pd_context* c)
+{
+    const int n = c->n;
+    const long nb = c->nb;
+    c->lpf = c->maxM <= 32 ? 32 : 64;
+    int rc = PD_OK;
+    const size_t wbytes = c->p.precision == PD_MIXED ? 8 : 16;
+    if ((rc = dalloc(c, &c->u, n)) || (rc = dalloc(c, &c->v, n)) || (rc = dalloc(c, &c->a, n)) ||
+        (rc = dalloc(c, &c->f, n)) || (rc = dalloc(c, &c->fown, n)) ||
+        (rc = dalloc(c, &c->kinv, 3 * (size_t)std::max(nb, 1L))) || (rc = dalloc(c, &c->broken, nb)) ||
+        (rc = dalloc(c, &c->pf1, 1)) || (rc = dalloc(c, (char**)&c->W, wbytes * (size_t)(nb > 0 ? nb : 1))) ||
+        (rc = dalloc(c, &c->counters, 4)) || (rc = dalloc(c, &c->bc, n)) || (rc = dalloc(c, &c->v0, 1)))
+        return rc;
+    hipError_t e = hipSuccess;
+    if (nb > 0) {
+        e = e != hipSuccess ? e : hipMemset(c->broken, 0, nb);
+        e = e != hipSuccess ? e : hipMemset(c->W, 0, wbytes * nb);
+    }
+    e = e != hipSuccess ? e : hipMemset(c->u, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->v, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->a, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->f, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->fown, 0, sizeof(double2) * n);
+    e = e != hipSuccess ? e : hipMemset(c->counters, 0, 16);
+    e = e != hipSuccess ? e : hipMemset(c->kinv, 0, sizeof(double) * 3 * std::max(nb, 1L));
+    if (nb == 0) {
+        // the bond kernel reads entry 0 of the bond arrays for idle lanes
+        e = e != hipSuccess ? e : hipMemset(c->col, 0, sizeof(int));
+        e = e != hipSuccess ? e : hipMemset(c->rev, 0, sizeof(int));
+        e = e != hipSuccess ? e : hipMemset(c->broken, 0, 1);
+    }
+    {
+        // plane-stress factor E/(1-nu^2) of getStiffnessTensor (src/main_seq.cpp:185)
+        const double pf = c->p.E / (1.0 - c->p.nu * c->p.nu);
+        e = e != hipSuccess ? e : hipMemcpy(c->pf1, &pf, sizeof(double), hipMemcpyHostToDevice);
+    }
+    {
+        std::vector<int> code(n, PD_BC_FREE);
+        e = e != hipSuccess ? e : hipMemcpy(c->bc, code.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    }
+    e = e != hipSuccess ? e : hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(c, PD_ERR_HIP, "initialisation failed: %s", hipGetErrorString(e));
+    return run_shape(c);
+}
+
 extern "C" int pd_create(const pd_params* p, int n, const double* X, const double* V,
                          const int* row_ptr, const int* col, pd_context** out)
 {
@@ -312,34 +383,14 @@ extern "C" int pd_create(const pd_params* p, int n, const double* X, const doubl
     }
 
     // --- device buffers ---
-    if (hipGetDevice(&c->dev) != hipSuccess) {
-        delete c;
-        return fail(nullptr, PD_ERR_HIP, "no HIP device available");
-    }
-    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess ||
-        c->n_cu <= 0)
-        c->n_cu = 256;
-    int rc = PD_OK;
-    const size_t wbytes = c->p.precision == PD_MIXED ? 8 : 16;
-    if ((rc = dalloc(c, &c->X, n)) || (rc = dalloc(c, &c->u, n)) || (rc = dalloc(c, &c->v, n)) ||
-        (rc = dalloc(c, &c->a, n)) || (rc = dalloc(c, &c->f, n)) || (rc = dalloc(c, &c->fown, n)) ||
-        (rc = dalloc(c, &c->V, n)) || (rc = dalloc(c, &c->row, n + 1)) ||
-        (rc = dalloc(c, &c->col, nb)) || (rc = dalloc(c, &c->rev, nb)) ||
-        (rc = dalloc(c, &c->kinv, 3 * (size_t)std::max(nb, 1L))) || (rc = dalloc(c, &c->broken, nb)) ||
-        (rc = dalloc(c, &c->pf1, 1)) ||
-        (rc = dalloc(c, (char**)&c->W, wbytes * (size_t)(nb > 0 ? nb : 1))) ||
-        (rc = dalloc(c, &c->counters, 4)) || (rc = dalloc(c, &c->bc, n)) ||
-        (rc = dalloc(c, &c->v0, 1))) {
+    auto bail = [&](int code) {
+        if (c->err.size()) g_err = c->err;
         free_all(c);
         delete c;
-        return rc;
-    }
-    auto bail = [&](int code) { free_all(c); delete c; return code; };
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(nullptr, PD_ERR_HIP, "hipStreamCreate failed"));
-    c->own_stream = true;
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
-        return bail(fail(nullptr, PD_ERR_HIP, "hipEventCreate failed"));
+        return code;
+    };
+    if (int rc = device_open(c)) return bail(rc);
+    if (int rc = alloc_mesh(c)) return bail(rc);
     hipError_t e = hipSuccess;
     e = e != hipSuccess ? e : hipMemcpy(c->X, Xi.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
     e = e != hipSuccess ? e : hipMemcpy(c->V, c->V_int.data(), sizeof(double) * n, hipMemcpyHostToDevice);
@@ -347,38 +398,9 @@ extern "C" int pd_create(const pd_params* p, int n, const double* X, const doubl
     if (nb > 0) {
         e = e != hipSuccess ? e : hipMemcpy(c->col, coli.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
         e = e != hipSuccess ? e : hipMemcpy(c->rev, rev.data(), sizeof(int) * nb, hipMemcpyHostToDevice);
-        e = e != hipSuccess ? e : hipMemset(c->broken, 0, nb);
-        e = e != hipSuccess ? e : hipMemset(c->W, 0, wbytes * nb);
     }
-    e = e != hipSuccess ? e : hipMemset(c->u, 0, sizeof(double2) * n);
-    e = e != hipSuccess ? e : hipMemset(c->v, 0, sizeof(double2) * n);
-    e = e != hipSuccess ? e : hipMemset(c->a, 0, sizeof(double2) * n);
-    e = e != hipSuccess ? e : hipMemset(c->f, 0, sizeof(double2) * n);
-    e = e != hipSuccess ? e : hipMemset(c->fown, 0, sizeof(double2) * n);
-    e = e != hipSuccess ? e : hipMemset(c->counters, 0, 16);
-    e = e != hipSuccess ? e : hipMemset(c->kinv, 0, sizeof(double) * 3 * std::max(nb, 1L));
-    if (nb == 0) {
-        // the bond kernel reads entry 0 of the bond arrays for idle lanes
-        e = e != hipSuccess ? e : hipMemset(c->col, 0, sizeof(int));
-        e = e != hipSuccess ? e : hipMemset(c->rev, 0, sizeof(int));
-        e = e != hipSuccess ? e : hipMemset(c->broken, 0, 1);
-    }
-    {
-        // plane-stress factor E/(1-nu^2) of getStiffnessTensor (src/main_seq.cpp:185)
-        const double pf = c->p.E / (1.0 - c->p.nu * c->p.nu);
-        e = e != hipSuccess ? e : hipMemcpy(c->pf1, &pf, sizeof(double), hipMemcpyHostToDevice);
-    }
-    e = e != hipSuccess ? e : hipDeviceSynchronize();
     if (e != hipSuccess) return bail(fail(nullptr, PD_ERR_HIP, "upload failed: %s", hipGetErrorString(e)));
-    {
-        std::vector<int> code(n, PD_BC_FREE);
-        if (hipMemcpy(c->bc, code.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess)
-            return bail(fail(nullptr, PD_ERR_HIP, "upload failed"));
-    }
-    if ((rc = run_shape(c))) {
-        g_err = c->err;
-        return bail(rc);
-    }
+    if (int rc = finish_create(c)) return bail(rc);
     *out = c;
     return PD_OK;
 }
@@ -399,6 +421,49 @@ extern "C" int pd_create_padded(const pd_params* p, int n, const double* X, cons
         row[i + 1] = (int)col.size();
     }
     return pd_create(p, n, X, V, row.data(), col.data(), out);
+}
+
+extern "C" int pd_create_from_points(const pd_params* p, int n, const double* X, const double* V,
+                                     pd_context** out)
+{
+    if (!out) return fail(nullptr, PD_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (int rc = check_params(p)) return rc;
+    if (n <= 0 || !X || !V) return fail(nullptr, PD_ERR_INVALID, "empty mesh or NULL array");
+    for (int i = 0; i < n; ++i)
+        if (!(V[i] > 0) || !std::isfinite(X[2 * i]) || !std::isfinite(X[2 * i + 1]))
+            return fail(nullptr, PD_ERR_INVALID, "particle %d: non-positive volume or non-finite position", i);
+    pd_context* c = new pd_context();
+    c->p = *p;
+    c->n = n;
+    c->n2_is_10 = (p->n2 == 10.0);
+    auto bail = [&](int code) {
+        if (c->err.size()) g_err = c->err;
+        free_all(c);
+        delete c;
+        return code;
+    };
+    if (int rc = device_open(c)) return bail(rc);
+    if (int rc = build_mesh_on_device(c, X, V)) return bail(rc);
+    if (int rc = finish_create(c)) return bail(rc);
+    *out = c;
+    return PD_OK;
+}
+
+extern "C" int pd_get_families(pd_context* c, int* row_ptr, int* col)
+{
+    if (!c || !row_ptr) return fail(c, PD_ERR_INVALID, "NULL argument");
+    hipSetDevice(c->dev);
+    // caller row lengths from the internal rows, then the caller's bond order
+    row_ptr[0] = 0;
+    for (int q = 0; q < c->n; ++q)
+        row_ptr[c->perm[q] + 1] = c->row_host[q + 1] - c->row_host[q];
+    for (int i = 0; i < c->n; ++i) row_ptr[i + 1] += row_ptr[i];
+    if (!col) return PD_OK;
+    std::vector<int> ci(std::max(c->nb, 1L));
+    if (c->nb) HIPCHK(c, hipMemcpy(ci.data(), c->col, sizeof(int) * c->nb, hipMemcpyDeviceToHost));
+    for (long b = 0; b < c->nb; ++b) col[c->bmap[b]] = c->perm[ci[b]];
+    return PD_OK;
 }
 
 extern "C" void pd_destroy(pd_context* c)

@@ -52,6 +52,7 @@ EXPORTS = (
     "pd_reset_profile", "pd_build_families", "pd_notch_mask", "pd_probe_copy",
     "pd_set_damage", "pd_compute_damage", "pd_get_damage", "pd_set_damage_state",
     "pd_read_mesh", "pd_write_velocity", "pd_write_positions", "pd_write_gathered",
+    "pd_create_from_points", "pd_get_families",
 )
 
 
@@ -149,6 +150,8 @@ def lib():
         "pd_get_damage": (I, [ctx, P, P, P]),
         "pd_set_damage_state": (I, [ctx, P, P]),
         "pd_read_mesh": (C.c_long, [C.c_char_p, I, P, P, C.c_long]),
+        "pd_create_from_points": (I, [C.POINTER(Params), I, P, P, C.POINTER(C.c_void_p)]),
+        "pd_get_families": (I, [ctx, P, P]),
         "pd_write_velocity": (I, [ctx, C.c_char_p, I]),
         "pd_write_positions": (I, [ctx, C.c_char_p, P, I]),
         "pd_write_gathered": (I, [ctx, C.c_char_p, P, I]),
@@ -231,6 +234,31 @@ class Context:
                              C.byref(self._h))
         if rc != PD_OK:
             raise PDError(rc, lib().pd_last_error(None).decode())
+
+    @classmethod
+    def from_points(cls, params, X, V):
+        """Families built on the device (pd_create_from_points); their CSR in
+        the caller's bond order is ``families()``."""
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        self.params = params
+        X = _f64(X)
+        V = _f64(V)
+        self.n = V.size
+        rc = lib().pd_create_from_points(C.byref(params), self.n, _ptr(X), _ptr(V), C.byref(self._h))
+        if rc != PD_OK:
+            raise PDError(rc, lib().pd_last_error(None).decode())
+        row = np.zeros(self.n + 1, dtype=np.int32)
+        self._check(lib().pd_get_families(self._h, _ptr(row), None))
+        self.nb = int(row[-1])
+        return self
+
+    def families(self):
+        """(row_ptr, col) in the caller's order."""
+        row = np.zeros(self.n + 1, dtype=np.int32)
+        col = np.zeros(max(self.nb, 1), dtype=np.int32)
+        self._check(lib().pd_get_families(self._h, _ptr(row), _ptr(col)))
+        return row, col[: self.nb]
 
     @classmethod
     def from_padded(cls, params, X, V, nbr, nbr_count, cap):

@@ -138,6 +138,18 @@ int pd_create(const pd_params* p, int n, const double* X, const double* V,
 int pd_create_padded(const pd_params* p, int n, const double* X, const double* V,
                      const int* nbr, const int* nbr_count, int cap, pd_context** out);
 
+/* Same, from particles alone: the families (findNeighbor's rule, cutoff
+ * |xi| < 1.01*dx*horizon, src/SetupParticleSystem.cpp:100-136, neighbours in
+ * ascending id order) are built on the device from a cell list, with the
+ * space-filling-curve order, internal CSR and reverse-bond index; the result
+ * equals pd_create with pd_build_families' families.  pd_get_families returns
+ * them (they define the caller's bond order of every per-bond array). */
+int pd_create_from_points(const pd_params* p, int n, const double* X, const double* V,
+                          pd_context** out);
+/* The context's families in the caller's order: row_ptr[n+1] and, when col is
+ * not NULL, col[nb] (nb = row_ptr[n]). */
+int pd_get_families(pd_context* ctx, int* row_ptr, int* col);
+
 void pd_destroy(pd_context* ctx);
 
 /* Work on the caller's HIP stream (hipStream_t); NULL = the context's own. */
