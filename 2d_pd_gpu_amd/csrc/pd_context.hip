@@ -936,35 +936,43 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     if (steps <= 0) steps = 5;
     hipSetDevice(c->dev);
     const size_t n = c->n, nb = c->nb;
+    const bool dmg = c->kappa != nullptr;
     double2 *su = nullptr, *sv = nullptr, *sa = nullptr, *sf = nullptr;
     unsigned char* sb = nullptr;
-    double* sk = nullptr;
+    double *sk = nullptr, *skap = nullptr, *sdm = nullptr, *sst = nullptr;
+    auto cleanup = [&]() {
+        for (void* q : {(void*)su, (void*)sv, (void*)sa, (void*)sf, (void*)sb, (void*)sk, (void*)skap,
+                        (void*)sdm, (void*)sst})
+            if (q) hipFree(q);
+    };
     int rc;
     if ((rc = dalloc(c, &su, n)) || (rc = dalloc(c, &sv, n)) || (rc = dalloc(c, &sa, n)) ||
-        (rc = dalloc(c, &sf, n)) || (rc = dalloc(c, &sb, nb)) || (rc = dalloc(c, &sk, 3 * nb)))
+        (rc = dalloc(c, &sf, n)) || (rc = dalloc(c, &sb, nb)) || (rc = dalloc(c, &sk, 3 * nb)) ||
+        (dmg && ((rc = dalloc(c, &skap, nb)) || (rc = dalloc(c, &sdm, nb)) || (rc = dalloc(c, &sst, n))))) {
+        cleanup();
         return rc;
+    }
+    // the variants are timed on the context's own state, restored after each
     auto save = [&](bool restore) -> hipError_t {
-        hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        const hipMemcpyKind k = hipMemcpyDeviceToDevice;
         hipError_t e = hipSuccess;
-        if (!restore) {
-            e = e ? e : hipMemcpyAsync(su, c->u, 16 * n, k, c->stream);
-            e = e ? e : hipMemcpyAsync(sv, c->v, 16 * n, k, c->stream);
-            e = e ? e : hipMemcpyAsync(sa, c->a, 16 * n, k, c->stream);
-            e = e ? e : hipMemcpyAsync(sf, c->f, 16 * n, k, c->stream);
-            if (nb) e = e ? e : hipMemcpyAsync(sb, c->broken, nb, k, c->stream);
-            if (nb) e = e ? e : hipMemcpyAsync(sk, c->kinv, 24 * nb, k, c->stream);
-        } else {
-            e = e ? e : hipMemcpyAsync(c->u, su, 16 * n, k, c->stream);
-            e = e ? e : hipMemcpyAsync(c->v, sv, 16 * n, k, c->stream);
-            e = e ? e : hipMemcpyAsync(c->a, sa, 16 * n, k, c->stream);
-            e = e ? e : hipMemcpyAsync(c->f, sf, 16 * n, k, c->stream);
-            if (nb) e = e ? e : hipMemcpyAsync(c->broken, sb, nb, k, c->stream);
-            if (nb) e = e ? e : hipMemcpyAsync(c->kinv, sk, 24 * nb, k, c->stream);
+        auto cp = [&](void* live, void* snap, size_t bytes) {
+            if (bytes && e == hipSuccess)
+                e = restore ? hipMemcpyAsync(live, snap, bytes, k, c->stream)
+                            : hipMemcpyAsync(snap, live, bytes, k, c->stream);
+        };
+        cp(c->u, su, 16 * n);
+        cp(c->v, sv, 16 * n);
+        cp(c->a, sa, 16 * n);
+        cp(c->f, sf, 16 * n);
+        cp(c->broken, sb, nb);
+        cp(c->kinv, sk, 24 * nb);
+        if (dmg) {
+            cp(c->kappa, skap, 8 * nb);
+            cp(c->dmg, sdm, 8 * nb);
+            cp(c->dstat, sst, 8 * n);
         }
-        return e ? e : hipStreamSynchronize(c->stream);
-    };
-    auto cleanup = [&]() {
-        hipFree(su); hipFree(sv); hipFree(sa); hipFree(sf); hipFree(sb); hipFree(sk);
+        return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
     };
     if (save(false) != hipSuccess) {
         cleanup();
@@ -973,7 +981,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     const int saved_profile = c->profile;
     const long saved_steps = c->steps;
     c->profile = 0;
-    struct Var { int wpb, xcd, lpf, bpl, mf, persist; };
+    struct Var { int wpb, xcd, lpf, bpl, mf, persist; float ms; };
     std::vector<Var> vars;
     const int mixed = c->p.precision == PD_MIXED;
     const int tiles[4][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}};
@@ -983,41 +991,50 @@ extern "C" int pd_autotune(pd_context* c, int steps)
             for (int w : {1, 2, 4, 8})
                 for (int x : {1, 8})
                     for (int pz : {0, 1, 2, 4})
-                        if (pz == 0 || w * pz <= 32) vars.push_back({w, x, tl[0], tl[1], mf, pz});
+                        if (pz == 0 || w * pz <= 32) vars.push_back({w, x, tl[0], tl[1], mf, pz, 0.f});
         }
-    double best = 1e300;
-    Var bestv = {c->wpb, c->xcd, c->lpf, c->bpl, c->mf, c->persist};
-    for (const Var& vv : vars) {
-        c->wpb = vv.wpb;
-        c->xcd = vv.xcd;
-        c->lpf = vv.lpf;
-        c->bpl = vv.bpl;
-        c->mf = vv.mf;
-        c->persist = vv.persist;
+    const Var keep = {c->wpb, c->xcd, c->lpf, c->bpl, c->mf, c->persist, 0.f};
+    auto use = [&](const Var& v) {
+        c->wpb = v.wpb;
+        c->xcd = v.xcd;
+        c->lpf = v.lpf;
+        c->bpl = v.bpl;
+        c->mf = v.mf;
+        c->persist = v.persist;
         c->graph_valid = false;
-        if ((rc = pd_step(c, 1))) break;  // warm-up
-        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        if ((rc = pd_step(c, steps))) break;
-        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-        HIPCHK(c, hipEventSynchronize(c->ev1));
+    };
+    auto time_var = [&](Var& v, int nsteps) -> int {
+        use(v);
+        if (int r = pd_step(c, 1)) return r;  // warm-up (graph capture)
+        hipError_t e = hipEventRecord(c->ev0, c->stream);
+        if (e == hipSuccess)
+            if (int r = pd_step(c, nsteps)) return r;
+        e = e != hipSuccess ? e : hipEventRecord(c->ev1, c->stream);
+        e = e != hipSuccess ? e : hipEventSynchronize(c->ev1);
         float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        if (ms < best) {
-            best = ms;
-            bestv = vv;
-        }
-        if (save(true) != hipSuccess) {
-            rc = fail(c, PD_ERR_HIP, "autotune: state restore failed");
-            break;
-        }
+        e = e != hipSuccess ? e : hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        e = e != hipSuccess ? e : save(true);
+        if (e != hipSuccess) return fail(c, PD_ERR_HIP, "autotune: %s", hipGetErrorString(e));
+        v.ms = ms / nsteps;
+        return PD_OK;
+    };
+    // pass 1: every variant for `steps` steps; pass 2: the four fastest again
+    // for 4x as many steps (short timings are noisy)
+    rc = PD_OK;
+    for (Var& v : vars)
+        if ((rc = time_var(v, steps))) break;
+    Var best = keep;
+    if (!rc && !vars.empty()) {
+        std::sort(vars.begin(), vars.end(), [](const Var& a, const Var& b) { return a.ms < b.ms; });
+        vars.resize(std::min<size_t>(4, vars.size()));
+        for (Var& v : vars)
+            if ((rc = time_var(v, 4 * steps))) break;
+        if (!rc)
+            best = *std::min_element(vars.begin(), vars.end(),
+                                     [](const Var& a, const Var& b) { return a.ms < b.ms; });
     }
-    c->wpb = bestv.wpb;
-    c->xcd = bestv.xcd;
-    c->lpf = bestv.lpf;
-    c->bpl = bestv.bpl;
-    c->mf = bestv.mf;
-    c->persist = bestv.persist;
-    c->graph_valid = false;
+    if (rc) save(true);
+    use(rc ? keep : best);
     c->profile = saved_profile;
     c->steps = saved_steps;
     cleanup();

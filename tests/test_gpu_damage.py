@@ -168,3 +168,21 @@ def test_adapter_compute_damage_status(tmp_path):
     assert np.linalg.norm(bonds[:, 1] - kap) <= 1e-10 * np.linalg.norm(kap)
     assert np.linalg.norm(bonds[:, 0] - dm) <= 1e-9 * np.linalg.norm(dm)
     assert np.linalg.norm(status - st) <= 1e-9 * np.linalg.norm(st)
+
+
+def test_autotune_restores_damage_state(pdpkg, plate):
+    say("autotune leaves damage state as it was")
+    pdh = pdpkg.pdhip
+    ctx = plate.context(s0=0.005)
+    ctx.set_damage(pdh.DamageParams.make(mode=pdh.DAMAGE_MAZARS, kappa0=KAPPA0))
+    ctx.step(40)
+    before = ctx.get_damage()
+    st0 = ctx.get_state()
+    ctx.autotune(2)
+    after = ctx.get_damage()
+    st1 = ctx.get_state()
+    ctx.close()
+    for key in ("bond", "kappa", "status"):
+        np.testing.assert_array_equal(before[key], after[key])
+    for key in ("u", "v", "a", "f"):
+        np.testing.assert_array_equal(st0[key], st1[key])
