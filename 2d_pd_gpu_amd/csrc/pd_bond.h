@@ -358,6 +358,17 @@ __device__ __forceinline__ int wave_max_family(int m)
     return __builtin_amdgcn_readfirstlane(m);
 }
 
+// K^-1 storage: FP64 values rounded to the bond arithmetic's precision (the
+// mixed-precision kernel rounds them to FP32 before use anyway).
+template <typename KT>
+__device__ __forceinline__ void store_kinv(const BondArgs& a, int b, const double* k)
+{
+    KT* d = reinterpret_cast<KT*>(a.kinv) + 3 * (size_t)b;
+    d[0] = KT(k[0]);
+    d[1] = KT(k[1]);
+    d[2] = KT(k[2]);
+}
+
 // K_ik^-1 for every bond of every family (context creation, pd_set_broken).
 template <int LPF, int BPL, int N2_IS_10>
 __global__ void __launch_bounds__(512) k_shape(BondArgs a)
@@ -386,9 +397,10 @@ __global__ void __launch_bounds__(512) k_shape(BondArgs a)
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         if (S[s].act) {
-            a.kinv[3 * (size_t)S[s].b] = kinv[s][0];
-            a.kinv[3 * (size_t)S[s].b + 1] = kinv[s][1];
-            a.kinv[3 * (size_t)S[s].b + 2] = kinv[s][2];
+            if (a.kinv_f32)
+                store_kinv<float>(a, S[s].b, kinv[s]);
+            else
+                store_kinv<double>(a, S[s].b, kinv[s]);
         }
         nsing += __popcll(__ballot(sing[s]));
     }
@@ -429,9 +441,10 @@ __device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<BPL>& L)
     }
 }
 
-template <int LPF, int BPL, int DMG = 0>
+template <typename R, int LPF, int BPL, int DMG = 0>
 __device__ __forceinline__ void stage_bonds(const BondArgs& a, FamLoad<BPL>& L)
 {
+    const R* kv = reinterpret_cast<const R*>(a.kinv);
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         BondSlot& S = L.S[s];
@@ -445,9 +458,9 @@ __device__ __forceinline__ void stage_bonds(const BondArgs& a, FamLoad<BPL>& L)
         S.Vj = a.V[j];
         L.br[s] = a.broken[b];
         L.rv[s] = a.rev[b];
-        L.kinv[s][0] = a.kinv[3 * b];
-        L.kinv[s][1] = a.kinv[3 * b + 1];
-        L.kinv[s][2] = a.kinv[3 * b + 2];
+        L.kinv[s][0] = double(kv[3 * b]);
+        L.kinv[s][1] = double(kv[3 * b + 1]);
+        L.kinv[s][2] = double(kv[3 * b + 2]);
         if constexpr (DMG) {
             L.kap[s] = a.kappa[b];
             L.dm[s] = a.dmg[b];
@@ -525,9 +538,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
             if (S[s].act) {
-                a.kinv[3 * (size_t)S[s].b] = kinv[s][0];
-                a.kinv[3 * (size_t)S[s].b + 1] = kinv[s][1];
-                a.kinv[3 * (size_t)S[s].b + 2] = kinv[s][2];
+                store_kinv<R>(a, S[s].b, kinv[s]);
             }
             nsing += __popcll(__ballot(sing[s]));
         }
@@ -587,13 +598,17 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
     // Families of 64 slots evaluate the weights again in contraction 2 rather
     // than keep a row of 64 weights live per lane: the occupancy gained
     // outweighs the extra VALU work there (measured: FP64 -24 %, mixed -4 %),
-    // not for 32-slot families (+7 % and more).
-    constexpr bool RW = !MF && FS >= 64;
+    // not for one-bond-per-lane 32-slot families (+7 % and more).
+    // NC of the lane's BPL weight rows are kept for contraction 2, the others
+    // are evaluated again there (FP64 two-bonds-per-lane: one row of 32 doubles
+    // fewer, measured -2.4 %)
+    constexpr int NC = (MF64 || (!MF && FS >= 64)) ? 0 : (sizeof(R) == 8 && BPL == 2) ? 1 : BPL;
+    constexpr bool RW = NC < BPL;
     // neighbours per scheduling group of the contraction loops (each group's
     // record loads are issued together; 2 frees no registers in the FP64
     // two-bonds-per-lane tile, whose live set is its two weight rows)
     constexpr int GL = 4;
-    R w[(MF64 || RW) ? 1 : BPL][(MF64 || RW) ? 1 : FS];  // per-lane weight rows otherwise
+    R w[NC > 0 ? NC : 1][NC > 0 ? FS : 1];  // per-lane weight rows
     R D[BPL][4];
     v4f Dv[BPL];
 #pragma unroll
@@ -671,7 +686,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
-                        if constexpr (!RW) w[s][l] = om;
+                        if (s < NC) w[s][l] = om;
                         Dv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Pa, om, Dv[s], 0, 0, 0);
                     }
                 } else if constexpr (COMPACT) {
@@ -680,7 +695,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
-                        if constexpr (!RW) w[s][l] = om;
+                        if (s < NC) w[s][l] = om;
                         const R t0 = om * a0, t1 = om * a1;
                         D[s][0] += t0 * ex;
                         D[s][1] += t0 * ey;
@@ -695,7 +710,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
-                        if constexpr (!RW) w[s][l] = om;
+                        if (s < NC) w[s][l] = om;
                         D[s][0] += om * P0;
                         D[s][1] += om * P1;
                         D[s][2] += om * P2;
@@ -708,7 +723,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
             for (int l = l0; l < l0 + GL; ++l)
 #pragma unroll
                 for (int s = 0; s < BPL; ++s)
-                    if constexpr (!RW) w[s][l] = R(0);
+                    if (s < NC) w[s][l] = R(0);
         }
         // keep the scheduler from hoisting every record load of the unrolled
         // loop (which would spill)
@@ -838,7 +853,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         R om;
-                        if constexpr (RW)
+                        if (s >= NC)
                             om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
                         else
                             om = w[s][l];
@@ -916,7 +931,7 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     FamLoad<BPL> L;
     stage_rows<LPF, BPL>(a, L, lane, blk * wpb + wib);
     stage_cols<LPF, BPL>(a, L);
-    stage_bonds<LPF, BPL, DMG>(a, L);
+    stage_bonds<R, LPF, BPL, DMG>(a, L);
     bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * Tile<LPF, BPL>::REC * 8, lane,
                                               NoHook(), NoHook());
 }
@@ -945,7 +960,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
     FamLoad<BPL> cur, nxt;
     stage_rows<LPF, BPL>(a, cur, lane, g);
     stage_cols<LPF, BPL>(a, cur);
-    stage_bonds<LPF, BPL, DMG>(a, cur);
+    stage_bonds<R, LPF, BPL, DMG>(a, cur);
     for (;;) {
         const int gn = g + wpb;
         const bool more = gn < g1;
@@ -957,7 +972,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
         bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(
             a, cur, rec64, lane,
             [&]() { stage_cols<LPF, BPL>(a, nxt); },
-            [&]() { stage_bonds<LPF, BPL, DMG>(a, nxt); });
+            [&]() { stage_bonds<R, LPF, BPL, DMG>(a, nxt); });
         if (!more) break;
         cur = nxt;
         g = gn;

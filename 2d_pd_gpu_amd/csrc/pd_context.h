@@ -37,7 +37,8 @@ struct pd_context {
     // device buffers
     double2 *X = nullptr, *u = nullptr, *v = nullptr, *a = nullptr, *f = nullptr, *fown = nullptr;
     double2* v0 = nullptr;
-    double *V = nullptr, *E = nullptr, *kinv = nullptr;
+    double *V = nullptr, *E = nullptr;
+    void* kinv = nullptr;    // 3 per bond, double (FP64) or float (mixed)
     double* pf1 = nullptr;   // the uniform plane-stress factor (one value)
     // Mazars bond damage
     pd_damage_params dmgp = {PD_DAMAGE_OFF, 0, 0, 0, 0, 0};

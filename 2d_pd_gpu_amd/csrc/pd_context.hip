@@ -52,6 +52,7 @@ static BondArgs bond_args(pd_context* c)
     a.u = c->u;
     a.broken = c->broken;
     a.kinv = c->kinv;
+    a.kinv_f32 = c->p.precision == PD_MIXED;
     a.W = c->W;
     a.fown = c->fown;
     // plane-stress factor E/(1-nu^2) of getStiffnessTensor (src/main_seq.cpp:185)
@@ -227,6 +228,12 @@ int alloc_mesh(pd_context* c)
     return rc;
 }
 
+// K^-1 storage: 3 values per bond in the bond arithmetic's precision.
+static size_t kinv_bytes(const pd_context* c)
+{
+    return 3 * (size_t)std::max(c->nb, 1L) * (c->p.precision == PD_MIXED ? sizeof(float) : sizeof(double));
+}
+
 // State and per-bond buffers, their initial values, and K^-1 of every bond.
 static int finish_create(pd_context* c)
 {
@@ -237,7 +244,7 @@ static int finish_create(pd_context* c)
     const size_t wbytes = c->p.precision == PD_MIXED ? 8 : 16;
     if ((rc = dalloc(c, &c->u, n)) || (rc = dalloc(c, &c->v, n)) || (rc = dalloc(c, &c->a, n)) ||
         (rc = dalloc(c, &c->f, n)) || (rc = dalloc(c, &c->fown, n)) ||
-        (rc = dalloc(c, &c->kinv, 3 * (size_t)std::max(nb, 1L))) || (rc = dalloc(c, &c->broken, nb)) ||
+        (rc = dalloc(c, (char**)&c->kinv, kinv_bytes(c))) || (rc = dalloc(c, &c->broken, nb)) ||
         (rc = dalloc(c, &c->pf1, 1)) || (rc = dalloc(c, (char**)&c->W, wbytes * (size_t)(nb > 0 ? nb : 1))) ||
         (rc = dalloc(c, &c->counters, 4)) || (rc = dalloc(c, &c->bc, n)) || (rc = dalloc(c, &c->v0, 1)))
         return rc;
@@ -252,7 +259,7 @@ static int finish_create(pd_context* c)
     e = e != hipSuccess ? e : hipMemset(c->f, 0, sizeof(double2) * n);
     e = e != hipSuccess ? e : hipMemset(c->fown, 0, sizeof(double2) * n);
     e = e != hipSuccess ? e : hipMemset(c->counters, 0, 16);
-    e = e != hipSuccess ? e : hipMemset(c->kinv, 0, sizeof(double) * 3 * std::max(nb, 1L));
+    e = e != hipSuccess ? e : hipMemset(c->kinv, 0, kinv_bytes(c));
     if (nb == 0) {
         // the bond kernel reads entry 0 of the bond arrays for idle lanes
         e = e != hipSuccess ? e : hipMemset(c->col, 0, sizeof(int));
@@ -947,7 +954,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     };
     int rc;
     if ((rc = dalloc(c, &su, n)) || (rc = dalloc(c, &sv, n)) || (rc = dalloc(c, &sa, n)) ||
-        (rc = dalloc(c, &sf, n)) || (rc = dalloc(c, &sb, nb)) || (rc = dalloc(c, &sk, 3 * nb)) ||
+        (rc = dalloc(c, &sf, n)) || (rc = dalloc(c, &sb, nb)) || (rc = dalloc(c, (char**)&sk, kinv_bytes(c))) ||
         (dmg && ((rc = dalloc(c, &skap, nb)) || (rc = dalloc(c, &sdm, nb)) || (rc = dalloc(c, &sst, n))))) {
         cleanup();
         return rc;
@@ -966,7 +973,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
         cp(c->a, sa, 16 * n);
         cp(c->f, sf, 16 * n);
         cp(c->broken, sb, nb);
-        cp(c->kinv, sk, 24 * nb);
+        cp(c->kinv, sk, kinv_bytes(c));
         if (dmg) {
             cp(c->kappa, skap, 8 * nb);
             cp(c->dmg, sdm, 8 * nb);

@@ -20,7 +20,9 @@ struct BondArgs {
     const int* rev;           // reverse bond of each bond
     const double2* u;         // displacement y - X
     unsigned char* broken;    // per bond, 1 = broken (irreversible)
-    double* kinv;             // per bond K_ik^-1 (xx, xy, yy), FP64
+    void* kinv;               // per bond K_ik^-1 (xx, xy, yy), assembled and inverted in
+                              // FP64, stored in the bond arithmetic's precision
+    int kinv_f32;             // 1: stored as float (mixed precision), 0: double
     void* W;                  // per bond, slot rev[b] <- T_ij V_i (double2 / float2)
     double2* fown;            // per particle sum_j T_ij V_j
     const double* pf;         // plane-stress factor E/(1-nu^2): pf[i & pf_mask]
