@@ -54,6 +54,7 @@ struct pd_context {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int use_graph = 1, profile = 0, wpb = 4, xcd = 8, lpf_opt = 0;
+    int check_finite = 1;    // pd_step reports a non-finite state (PD_OPT_CHECK_FINITE)
     int persist = 0;  // persistent bond kernel: workgroups per CU (0 = one wave per group)
     int n_cu = 256;
     hipGraph_t graph = nullptr;

@@ -526,6 +526,7 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
         c->wpb = (int)value;
         break;
     case PD_OPT_XCD_REMAP: c->xcd = value ? 8 : 1; break;
+    case PD_OPT_CHECK_FINITE: c->check_finite = value ? 1 : 0; break;
     case PD_OPT_PERSISTENT:
         if (value < 0 || value > 64) return fail(c, PD_ERR_INVALID, "persistent workgroups per CU must be in [0, 64]");
         c->persist = (int)value;
@@ -834,7 +835,15 @@ extern "C" int pd_step(pd_context* c, int nsteps)
             if (int rc = enqueue_step(c)) return rc;
     }
     c->steps += nsteps;
+    unsigned int first = 0xffffffffu;
+    if (c->check_finite) {
+        HIPCHK(c, launch_check_finite(c->n, c->u, c->v, c->f, c->counters + 2, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&first, c->counters + 2, sizeof(first), hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (first < (unsigned int)c->n)
+        return fail(c, PD_ERR_NOT_FINITE, "particle %d has a non-finite displacement, velocity or force "
+                    "after step %ld", c->perm[first], c->steps);
     return PD_OK;
 }
 

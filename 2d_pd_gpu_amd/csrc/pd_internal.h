@@ -34,7 +34,7 @@ struct BondArgs {
     double brk_c;             // (1 + s0)^2
     double ktol;              // singular-K tolerance
     int force_k;              // 1: assemble K for every family in this launch
-    unsigned int* counters;   // [0] newly broken bonds, [1] singular K
+    unsigned int* counters;   // [0] newly broken bonds, [1] singular K, [2] first non-finite
     // Mazars bond damage (kernels built with damage only)
     int dmg_mode;             // 1: running threshold in the law (reference), 2: initial
     double kap0, At, Bt, Ac, Bc;
@@ -89,6 +89,9 @@ hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double
                                    double2* a, hipStream_t s);
 hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
+// *first = smallest index with a non-finite u, v or f (0xffffffff if none).
+hipError_t launch_check_finite(int n, const double2* u, const double2* v, const double2* f,
+                               unsigned int* first, hipStream_t s);
 hipError_t launch_copy_probe(const double4* src, double4* dst, long n4, hipStream_t s);
 
 }  // namespace pd

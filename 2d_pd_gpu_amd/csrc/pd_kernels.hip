@@ -93,6 +93,19 @@ __global__ void k_fixed_list(int count, const int* ids, double2* u)
     u[ids[t]] = make_double2(0.0, 0.0);
 }
 
+// Smallest internal index whose u, v or f is not finite (integer atomic).
+__global__ void __launch_bounds__(256) k_check_finite(int n, const double2* __restrict__ u,
+                                                      const double2* __restrict__ v,
+                                                      const double2* __restrict__ f,
+                                                      unsigned int* __restrict__ first)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 a = u[i], b = v[i], c = f[i];
+    if (!(isfinite(a.x) && isfinite(a.y) && isfinite(b.x) && isfinite(b.y) && isfinite(c.x) && isfinite(c.y)))
+        atomicMin(first, (unsigned int)i);
+}
+
 __global__ void k_fill_u32(unsigned int* p, unsigned int v, int count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,6 +193,15 @@ hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fixed_list, dim3(ceil_div(count, 256)), dim3(256), 0, s, count, ids, u);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_finite(int n, const double2* u, const double2* v, const double2* f,
+                               unsigned int* first, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, first, 0xffffffffu, 1);
+    hipLaunchKernelGGL(k_check_finite, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, u, v, f, first);
     return hipGetLastError();
 }
 

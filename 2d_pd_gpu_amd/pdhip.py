@@ -36,6 +36,7 @@ OPT_WAVES_PER_WG = 3
 OPT_XCD_REMAP = 4
 OPT_TILE = 5
 OPT_PERSISTENT = 6
+OPT_CHECK_FINITE = 7
 
 DAMAGE_OFF = 0
 DAMAGE_REFERENCE = 1
@@ -52,7 +53,7 @@ EXPORTS = (
     "pd_reset_profile", "pd_build_families", "pd_notch_mask", "pd_probe_copy",
     "pd_set_damage", "pd_compute_damage", "pd_get_damage", "pd_set_damage_state",
     "pd_read_mesh", "pd_write_velocity", "pd_write_positions", "pd_write_gathered",
-    "pd_create_from_points", "pd_get_families",
+    "pd_create_from_points", "pd_get_families", "pd_save_checkpoint", "pd_load_checkpoint",
 )
 
 
@@ -152,6 +153,8 @@ def lib():
         "pd_read_mesh": (C.c_long, [C.c_char_p, I, P, P, C.c_long]),
         "pd_create_from_points": (I, [C.POINTER(Params), I, P, P, C.POINTER(C.c_void_p)]),
         "pd_get_families": (I, [ctx, P, P]),
+        "pd_save_checkpoint": (I, [ctx, C.c_char_p]),
+        "pd_load_checkpoint": (I, [ctx, C.c_char_p]),
         "pd_write_velocity": (I, [ctx, C.c_char_p, I]),
         "pd_write_positions": (I, [ctx, C.c_char_p, P, I]),
         "pd_write_gathered": (I, [ctx, C.c_char_p, P, I]),
@@ -404,6 +407,13 @@ class Context:
     def write_gathered(self, path, ids=None, digits=0):
         ids = None if ids is None else _i32(ids)
         self._check(lib().pd_write_gathered(self._h, os.fsencode(path), _ptr(ids), int(digits)))
+
+    # -- checkpoint / resume --
+    def save_checkpoint(self, path):
+        self._check(lib().pd_save_checkpoint(self._h, os.fsencode(path)))
+
+    def load_checkpoint(self, path):
+        self._check(lib().pd_load_checkpoint(self._h, os.fsencode(path)))
 
     def stats(self):
         s = Stats()

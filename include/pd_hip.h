@@ -73,6 +73,9 @@ extern "C" {
 #define PD_OPT_PERSISTENT    6  /* bond kernel as a persistent grid of value workgroups
                                    per CU that prefetch their next family group
                                    (0 = one wave per family group, the default)      */
+#define PD_OPT_CHECK_FINITE  7  /* 1: pd_step returns PD_ERR_NOT_FINITE, naming the
+                                   first particle, when u, v or f is not finite after
+                                   its steps (default 1; one reduction per call)     */
 
 /* ---- Mazars bond damage (pd_set_damage) ------------------------------------ */
 #define PD_DAMAGE_OFF        0
@@ -247,6 +250,14 @@ int pd_write_positions(pd_context* ctx, const char* path, const int* ids, int di
  * file of a one-GPU run: "ID, x, y, ux, uy, damage" with damageStatus (0 when
  * damage was never enabled). */
 int pd_write_gathered(pd_context* ctx, const char* path, const int* ids, int digits);
+
+/* ---- checkpoint / resume (no counterpart in the reference) -------------------
+ * Binary file of the state the next step reads: u, v, a, f, the broken mask,
+ * the damage state and the step count, in the caller's order, with a hash of
+ * the mesh.  A context of the same mesh and precision that loads it continues
+ * bit for bit like the run that wrote it. */
+int pd_save_checkpoint(pd_context* ctx, const char* path);
+int pd_load_checkpoint(pd_context* ctx, const char* path);
 
 /* ---- diagnostics (no counterpart in the reference) ---------------------------
  * Times a plain device-to-device copy kernel that reads and writes `bytes`

@@ -93,3 +93,39 @@ def test_gathered_file_with_damage(pdpkg, tmp_path):
     np.testing.assert_array_equal(tab[:, 3:5].reshape(-1), u * 2e4)
     np.testing.assert_array_equal(tab[:, 5], status)
     assert status.max() > 0
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_checkpoint_resume_is_bitwise(pdpkg, tmp_path, precision):
+    """30 steps, checkpoint, 30 more, against a fresh context that loads the
+    checkpoint and runs the last 30: identical state, broken set and damage."""
+    say(f"checkpoint / resume, precision {precision}")
+    pdh = pdpkg.pdhip
+    w = pdpkg.plates.single_plate("p48x40", 48, 40, 30.0, notch_len_cells=16, seed=3)
+    ck = tmp_path / "state.ckpt"
+    ctx = w.context(precision=precision, s0=0.004)
+    ctx.set_damage(pdh.DamageParams.make(mode=pdh.DAMAGE_MAZARS, kappa0=1e-4))
+    ctx.step(30)
+    ctx.save_checkpoint(str(ck))
+    ctx.step(30)
+    a_state, a_br, a_dmg, a_steps = ctx.get_state(), ctx.get_broken(), ctx.get_damage(), ctx.stats().steps
+    ctx.close()
+    ctx = w.context(precision=precision, s0=0.004)
+    ctx.load_checkpoint(str(ck))
+    assert ctx.stats().steps == 30
+    ctx.step(30)
+    b_state, b_br, b_dmg = ctx.get_state(), ctx.get_broken(), ctx.get_damage()
+    assert ctx.stats().steps == a_steps
+    ctx.close()
+    assert a_br.sum() > w.broken.sum()
+    for k in ("u", "v", "a", "f"):
+        np.testing.assert_array_equal(a_state[k], b_state[k])
+    np.testing.assert_array_equal(a_br, b_br)
+    for k in ("bond", "kappa", "status"):
+        np.testing.assert_array_equal(a_dmg[k], b_dmg[k])
+    # a checkpoint of another mesh is refused
+    other = pdpkg.plates.single_plate("o", 40, 40, 30.0, seed=4)
+    ctx = other.context(precision=precision)
+    with pytest.raises(pdh.PDError):
+        ctx.load_checkpoint(str(ck))
+    ctx.close()

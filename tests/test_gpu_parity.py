@@ -418,3 +418,45 @@ def test_batched_plates_equal_individual_runs(pdpkg):
     u, v, f, _ = run_plate_oracle(wb, 0.0, 40)
     _, nonbc, _ = H.parity_norms(st["v"], v, wb.code >= 0)
     assert nonbc < 1e-11
+
+
+def test_non_finite_state_is_reported(pdpkg, small_plate):
+    say("non-finite state -> PD_ERR_NOT_FINITE naming the particle")
+    pdh = pdpkg.pdhip
+    ctx = small_plate.context(s0=0.0)
+    u = np.zeros(2 * small_plate.n)
+    u[2 * 17 + 1] = np.nan
+    ctx.set_state(u=u)
+    with pytest.raises(pdh.PDError) as e:
+        ctx.step(2)
+    assert e.value.code == pdh.PD_ERR_NOT_FINITE
+    # the named particle (the NaN has spread through the families by now) is bad
+    bad = int(str(e.value).split("particle ")[1].split()[0])
+    st = ctx.get_state()
+    assert not all(np.isfinite(st[k][2 * bad:2 * bad + 2]).all() for k in ("u", "v", "f"))
+    ctx.set_option(pdh.OPT_CHECK_FINITE, 0)
+    ctx.step(1)  # unchecked: runs, the NaN spreads
+    assert not np.isfinite(ctx.get_state()["v"]).all()
+    ctx.close()
+
+
+def test_collinear_families_are_singular_not_nan(pdpkg):
+    """A single row of particles: every K_ik is rank one, so every bond is
+    singular (K^-1 = 0), no stress, no NaN."""
+    say("collinear families: singular K guard")
+    pdh = pdpkg.pdhip
+    n = 40
+    X = np.zeros(2 * n)
+    X[0::2] = np.arange(n) * 1e-3
+    V = np.full(n, 1e-9)
+    row, col = pdh.build_families(X, 1e-3, 3.015)
+    ctx = pdh.Context(pdh.Params.make(s0=0.0), X, V, row, col)
+    st = ctx.stats()
+    assert st.n_singular == len(col)
+    code = np.full(n, pdh.BC_FREE, np.int32)
+    code[-1] = 0
+    ctx.set_boundary(code, np.array([1.0, 0.0]))
+    ctx.step(10)
+    s = ctx.get_state()
+    ctx.close()
+    assert np.isfinite(s["v"]).all() and np.abs(s["f"]).max() == 0.0
