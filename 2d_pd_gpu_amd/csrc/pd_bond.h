@@ -408,63 +408,78 @@ __global__ void __launch_bounds__(512) k_shape(BondArgs a)
 }
 
 // Global inputs of one family group for one lane, gathered in three dependent
-// stages (row pointers -> neighbour ids -> neighbour state, broken flags and
-// K^-1), so that a persistent wave can issue each stage for its next group
-// while it computes the current one.
-template <int BPL> struct FamLoad {
+// stages: row pointers; then everything indexed by the bond (neighbour id,
+// reverse bond, broken flag, K^-1, damage state); then the neighbours' state.
+// The stages only issue loads into these raw fields; the values are first
+// used in finish_loads at the start of the group's body, so that a persistent
+// wave can keep its next group's loads in flight across the current group.
+template <typename R, int BPL> struct FamLoad {
     LaneFam F;
     BondSlot S[BPL];
     int jr[BPL];              // neighbour id as loaded (valid where S.act)
     int rv[BPL];              // reverse bond
     unsigned char br[BPL];
-    double kinv[BPL][3];
+    R kraw[BPL][3];           // K^-1 as stored
+    double2 Xj[BPL], uj[BPL];
+    double Vj[BPL];
     double kap[BPL], dm[BPL];  // damage threshold and value (damage kernels)
 };
 
-template <int LPF, int BPL>
-__device__ __forceinline__ void stage_rows(const BondArgs& a, FamLoad<BPL>& L, int lane, int group)
+template <int LPF, typename FL>
+__device__ __forceinline__ void stage_rows(const BondArgs& a, FL& L, int lane, int group)
 {
     L.F = load_family<LPF>(a, lane, group);
 }
 
-template <int LPF, int BPL>
-__device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<BPL>& L)
+template <typename R, int LPF, int BPL, int DMG = 0>
+__device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<R, BPL>& L)
 {
     family_extent(L.F);
+    const R* kv = reinterpret_cast<const R*>(a.kinv);
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         BondSlot& S = L.S[s];
         S.k = L.F.t + s * LPF;
         S.act = L.F.famok && S.k < L.F.M;
         S.b = L.F.beg + S.k;
-        L.jr[s] = a.col[S.act ? S.b : 0];  // the bond arrays hold at least one entry
-    }
-}
-
-template <typename R, int LPF, int BPL, int DMG = 0>
-__device__ __forceinline__ void stage_bonds(const BondArgs& a, FamLoad<BPL>& L)
-{
-    const R* kv = reinterpret_cast<const R*>(a.kinv);
-#pragma unroll
-    for (int s = 0; s < BPL; ++s) {
-        BondSlot& S = L.S[s];
-        const int j = S.act ? L.jr[s] : (L.F.famok ? L.F.fam : 0);
-        const size_t b = S.act ? (size_t)S.b : 0;
-        const double2 Xj = a.X[j], uj = a.u[j];
-        S.x0 = Xj.x - L.F.Xi.x;
-        S.x1 = Xj.y - L.F.Xi.y;
-        S.d0 = uj.x - L.F.ui.x;
-        S.d1 = uj.y - L.F.ui.y;
-        S.Vj = a.V[j];
+        const size_t b = S.act ? (size_t)S.b : 0;  // the bond arrays hold at least one entry
+        L.jr[s] = a.col[b];
         L.br[s] = a.broken[b];
         L.rv[s] = a.rev[b];
-        L.kinv[s][0] = double(kv[3 * b]);
-        L.kinv[s][1] = double(kv[3 * b + 1]);
-        L.kinv[s][2] = double(kv[3 * b + 2]);
+        L.kraw[s][0] = kv[3 * b];
+        L.kraw[s][1] = kv[3 * b + 1];
+        L.kraw[s][2] = kv[3 * b + 2];
         if constexpr (DMG) {
             L.kap[s] = a.kappa[b];
             L.dm[s] = a.dmg[b];
         }
+    }
+}
+
+template <typename R, int LPF, int BPL>
+__device__ __forceinline__ void stage_bonds(const BondArgs& a, FamLoad<R, BPL>& L)
+{
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        const int j = L.S[s].act ? L.jr[s] : (L.F.famok ? L.F.fam : 0);
+        L.Xj[s] = a.X[j];
+        L.uj[s] = a.u[j];
+        L.Vj[s] = a.V[j];
+    }
+}
+
+// First use of the loaded values: bond vector, displacement difference, volume.
+template <typename R, int BPL>
+__device__ __forceinline__ void finish_loads(FamLoad<R, BPL>& L)
+{
+#pragma unroll
+    for (int s = 0; s < BPL; ++s) {
+        BondSlot& S = L.S[s];
+        S.x0 = L.Xj[s].x - L.F.Xi.x;
+        S.x1 = L.Xj[s].y - L.F.Xi.y;
+        S.d0 = L.uj[s].x - L.F.ui.x;
+        S.d1 = L.uj[s].y - L.F.ui.y;
+        S.Vj = L.Vj[s];
     }
 }
 
@@ -498,9 +513,10 @@ template <typename R, int MF> struct Contraction {
 // and the force-state outputs.  hook_a runs halfway through contraction 1,
 // hook_b after it (a persistent wave issues its next group's loads there).
 template <typename R, int LPF, int BPL, int N2_IS_10, int MF, int DMG, typename HA, typename HB>
-__device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, double* rec64,
+__device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L, double* rec64,
                                           int lane, const HA& hook_a, const HB& hook_b)
 {
+    finish_loads(L);
     constexpr bool MF32 = Contraction<R, MF>::MF32;
     constexpr bool MF64 = Contraction<R, MF>::MF64;
     static_assert(!MF64 || (BPL == 1 && (LPF == 32 || LPF == 64)),
@@ -546,9 +562,9 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<BPL>& L, do
     } else {
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
-            kinv[s][0] = S[s].act ? L.kinv[s][0] : 0.0;
-            kinv[s][1] = S[s].act ? L.kinv[s][1] : 0.0;
-            kinv[s][2] = S[s].act ? L.kinv[s][2] : 0.0;
+            kinv[s][0] = S[s].act ? double(L.kraw[s][0]) : 0.0;
+            kinv[s][1] = S[s].act ? double(L.kraw[s][1]) : 0.0;
+            kinv[s][2] = S[s].act ? double(L.kraw[s][2]) : 0.0;
         }
     }
 
@@ -928,10 +944,10 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     const int wib = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
     const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
-    FamLoad<BPL> L;
-    stage_rows<LPF, BPL>(a, L, lane, blk * wpb + wib);
-    stage_cols<LPF, BPL>(a, L);
-    stage_bonds<R, LPF, BPL, DMG>(a, L);
+    FamLoad<R, BPL> L;
+    stage_rows<LPF>(a, L, lane, blk * wpb + wib);
+    stage_cols<R, LPF, BPL, DMG>(a, L);
+    stage_bonds<R, LPF, BPL>(a, L);
     bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * Tile<LPF, BPL>::REC * 8, lane,
                                               NoHook(), NoHook());
 }
@@ -957,10 +973,10 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
     double* rec64 = smem + wib * Tile<LPF, BPL>::REC * 8;
     int g = g0 + wib;
     if (g >= g1) return;
-    FamLoad<BPL> cur, nxt;
-    stage_rows<LPF, BPL>(a, cur, lane, g);
-    stage_cols<LPF, BPL>(a, cur);
-    stage_bonds<R, LPF, BPL, DMG>(a, cur);
+    FamLoad<R, BPL> cur, nxt;
+    stage_rows<LPF>(a, cur, lane, g);
+    stage_cols<R, LPF, BPL, DMG>(a, cur);
+    stage_bonds<R, LPF, BPL>(a, cur);
     for (;;) {
         const int gn = g + wpb;
         const bool more = gn < g1;
@@ -968,11 +984,11 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
         // unconditional, so waits on the current group's data are counted
         // exactly instead of draining the prefetch
         const int gp = more ? gn : g;
-        stage_rows<LPF, BPL>(a, nxt, lane, gp);
+        stage_rows<LPF>(a, nxt, lane, gp);
         bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(
             a, cur, rec64, lane,
-            [&]() { stage_cols<LPF, BPL>(a, nxt); },
-            [&]() { stage_bonds<R, LPF, BPL, DMG>(a, nxt); });
+            [&]() { stage_cols<R, LPF, BPL, DMG>(a, nxt); },
+            [&]() { stage_bonds<R, LPF, BPL>(a, nxt); });
         if (!more) break;
         cur = nxt;
         g = gn;
