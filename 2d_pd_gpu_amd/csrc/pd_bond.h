@@ -145,19 +145,55 @@ template <> struct Vec2<float> { typedef float2 T; };
 template <int LPF, int BPL> struct Tile {
     static constexpr int FS = LPF * BPL;
     static constexpr int FPW = 64 / LPF;
-    static constexpr int REC = 64 * BPL;  // LDS records (8 values each) per wave
+    // LDS per wave, in doubles: FS rows of FPW FP64 records (8 values each)
+    // plus one 16-byte pad chunk per row (see rec_at)
+    static constexpr int LDS_DOUBLES = FS * (4 * FPW + 1) * 2;
 };
 
-// LDS record layout.  Value e (0..7) of record k of family `slot` sits in
-// 16-byte chunk e / VPC of that record; chunks are interleaved across the
-// wave's FPW families, so one broadcast read (every lane of a family asks for
-// the same chunk) touches FPW consecutive 16-byte words in distinct banks.
+// LDS record layout.  Row k holds record k of each of the wave's FPW
+// families; value e (0..7) sits in 16-byte chunk e / VPC, and the chunks are
+// interleaved across the families, so one broadcast read (every lane of a
+// family asks for the same chunk) touches FPW consecutive 16-byte words in
+// distinct banks.  A row is an odd number of chunks long (one pad chunk), so
+// the record writes, where the lanes of a family write consecutive rows, fall
+// into distinct 16-byte bank groups instead of all into one.
 template <typename T, int FPW>
 __device__ __forceinline__ int rec_at(int k, int slot, int e)
 {
     constexpr int VPC = 16 / (int)sizeof(T);
     constexpr int NCH = 8 / VPC;
-    return ((k * NCH + e / VPC) * FPW + slot) * VPC + e % VPC;
+    constexpr int ROW = (NCH * FPW + 1) * VPC;
+    return k * ROW + ((e / VPC) * FPW + slot) * VPC + e % VPC;
+}
+
+// Whole-chunk record writes (one 16-byte LDS store per chunk c).
+template <int FPW>
+__device__ __forceinline__ void rec_put(float* rec, int k, int slot, int c, float v0, float v1, float v2,
+                                        float v3)
+{
+    *reinterpret_cast<float4*>(rec + rec_at<float, FPW>(k, slot, 4 * c)) = make_float4(v0, v1, v2, v3);
+}
+
+template <int FPW>
+__device__ __forceinline__ void rec_put(double* rec, int k, int slot, int c, double v0, double v1)
+{
+    *reinterpret_cast<double2*>(rec + rec_at<double, FPW>(k, slot, 2 * c)) = make_double2(v0, v1);
+}
+
+// Values 4h .. 4h + 3 of record k.
+template <int FPW>
+__device__ __forceinline__ void rec_put4(float* rec, int k, int slot, int h, float v0, float v1, float v2,
+                                         float v3)
+{
+    rec_put<FPW>(rec, k, slot, h, v0, v1, v2, v3);
+}
+
+template <int FPW>
+__device__ __forceinline__ void rec_put4(double* rec, int k, int slot, int h, double v0, double v1,
+                                         double v2, double v3)
+{
+    rec_put<FPW>(rec, k, slot, 2 * h, v0, v1);
+    rec_put<FPW>(rec, k, slot, 2 * h + 1, v2, v3);
 }
 
 // The family a lane works on.  Loads are issued without branches (indices
@@ -292,7 +328,7 @@ template <> struct GeoOf<float> {
 
 // FP64 assembly of K_ik over the alive bonds of each family in the wave, and
 // its inverse under the scale-relative singularity rule.  rec: this wave's LDS
-// records (Tile::REC x 8 doubles).  Every lane of the wave must call it.
+// records (Tile::LDS_DOUBLES).  Every lane of the wave must call it.
 template <int LPF, int BPL, int N2_IS_10>
 __device__ __forceinline__ void assemble_kinv(double* rec, const LaneFam& F, const BondSlot* S,
                                               const bool* alive, const Geo<double>* g, double n2,
@@ -304,13 +340,10 @@ __device__ __forceinline__ void assemble_kinv(double* rec, const LaneFam& F, con
     for (int s = 0; s < BPL; ++s) {
         const int k = S[s].k;
         const double V = alive[s] ? S[s].Vj : 0.0;
-        rec[rec_at<double, FPW>(k, F.slot, 0)] = g[s].ex;
-        rec[rec_at<double, FPW>(k, F.slot, 1)] = g[s].ey;
-        rec[rec_at<double, FPW>(k, F.slot, 2)] = g[s].pe;
-        rec[rec_at<double, FPW>(k, F.slot, 3)] = g[s].qe;
-        rec[rec_at<double, FPW>(k, F.slot, 4)] = V * S[s].x0 * S[s].x0;
-        rec[rec_at<double, FPW>(k, F.slot, 5)] = V * S[s].x0 * S[s].x1;
-        rec[rec_at<double, FPW>(k, F.slot, 6)] = V * S[s].x1 * S[s].x1;
+        rec_put<FPW>(rec, k, F.slot, 0, g[s].ex, g[s].ey);
+        rec_put<FPW>(rec, k, F.slot, 1, g[s].pe, g[s].qe);
+        rec_put<FPW>(rec, k, F.slot, 2, V * S[s].x0 * S[s].x0, V * S[s].x0 * S[s].x1);
+        rec_put<FPW>(rec, k, F.slot, 3, V * S[s].x1 * S[s].x1, 0.0);
     }
     wave_sync();
     double K[BPL][3];
@@ -391,7 +424,7 @@ __global__ void __launch_bounds__(512) k_shape(BondArgs a)
     }
     double kinv[BPL][3];
     bool sing[BPL];
-    assemble_kinv<LPF, BPL, N2_IS_10>(smem + wib * Tile<LPF, BPL>::REC * 8, F, S, alive, g, a.n2,
+    assemble_kinv<LPF, BPL, N2_IS_10>(smem + wib * Tile<LPF, BPL>::LDS_DOUBLES, F, S, alive, g, a.n2,
                                       a.ktol, kinv, sing);
     int nsing = 0;
 #pragma unroll
@@ -579,22 +612,15 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     for (int s = 0; s < BPL; ++s) {
         g[s] = GeoOf<R>::get(a, S[s]);
         const int k = S[s].k;
-        rec[rec_at<R, FPW>(k, F.slot, 0)] = g[s].ex;
-        rec[rec_at<R, FPW>(k, F.slot, 1)] = g[s].ey;
-        rec[rec_at<R, FPW>(k, F.slot, 2)] = g[s].pe;
-        rec[rec_at<R, FPW>(k, F.slot, 3)] = g[s].qe;
+        rec_put4<FPW>(rec, k, F.slot, 0, g[s].ex, g[s].ey, g[s].pe, g[s].qe);
         if constexpr (COMPACT) {
             const double Vl = alive[s] ? S[s].Vj * double(g[s].len) : 0.0;
-            rec[rec_at<R, FPW>(k, F.slot, 4)] = R(Vl * S[s].d0);
-            rec[rec_at<R, FPW>(k, F.slot, 5)] = R(Vl * S[s].d1);
+            rec_put<FPW>(rec, k, F.slot, 2, R(Vl * S[s].d0), R(Vl * S[s].d1));
         } else {
             const R x0r = R(S[s].x0), x1r = R(S[s].x1);
             const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) : R(0);
             const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) : R(0);
-            rec[rec_at<R, FPW>(k, F.slot, 4)] = Vd0 * x0r;
-            rec[rec_at<R, FPW>(k, F.slot, 5)] = Vd0 * x1r;
-            rec[rec_at<R, FPW>(k, F.slot, 6)] = Vd1 * x0r;
-            rec[rec_at<R, FPW>(k, F.slot, 7)] = Vd1 * x1r;
+            rec_put4<FPW>(rec, k, F.slot, 1, Vd0 * x0r, Vd0 * x1r, Vd1 * x0r, Vd1 * x1r);
         }
     }
     // records past a family's end are zero (inactive slots), so the loops may
@@ -801,10 +827,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         const int k = S[s].k;
-        rec[rec_at<R, FPW>(k, F.slot, 4)] = Q[s][0];
-        rec[rec_at<R, FPW>(k, F.slot, 5)] = Q[s][1];
-        rec[rec_at<R, FPW>(k, F.slot, 6)] = Q[s][2];
-        rec[rec_at<R, FPW>(k, F.slot, 7)] = Q[s][3];
+        rec_put4<FPW>(rec, k, F.slot, 1, Q[s][0], Q[s][1], Q[s][2], Q[s][3]);
     }
     wave_sync();
 
@@ -948,7 +971,7 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     stage_rows<LPF>(a, L, lane, blk * wpb + wib);
     stage_cols<R, LPF, BPL, DMG>(a, L);
     stage_bonds<R, LPF, BPL>(a, L);
-    bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * Tile<LPF, BPL>::REC * 8, lane,
+    bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * Tile<LPF, BPL>::LDS_DOUBLES, lane,
                                               NoHook(), NoHook());
 }
 
@@ -970,7 +993,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
     const int per = ngroups / (int)gridDim.x, rem = ngroups % (int)gridDim.x;
     const int g0 = blk * per + (blk < rem ? blk : rem);
     const int g1 = g0 + per + (blk < rem ? 1 : 0);
-    double* rec64 = smem + wib * Tile<LPF, BPL>::REC * 8;
+    double* rec64 = smem + wib * Tile<LPF, BPL>::LDS_DOUBLES;
     int g = g0 + wib;
     if (g >= g1) return;
     FamLoad<R, BPL> cur, nxt;

@@ -148,7 +148,7 @@ static hipError_t launch_shape_t(const BondArgs& a, int wpb, hipStream_t s)
 {
     const int grid = ceil_div(a.n, (long)wpb * Tile<LPF, BPL>::FPW);
     if (grid == 0) return hipSuccess;
-    const size_t lds = (size_t)wpb * Tile<LPF, BPL>::REC * 8 * sizeof(double);
+    const size_t lds = (size_t)wpb * Tile<LPF, BPL>::LDS_DOUBLES * sizeof(double);
     hipLaunchKernelGGL((k_shape<LPF, BPL, N2>), dim3(grid), dim3(64 * wpb), lds, s, a);
     return hipGetLastError();
 }
