@@ -602,10 +602,11 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     }
 
     // ---- records for the first contraction ----
-    // The VALU FP64 contraction sends V l du and reuses the record's unit vector
-    // (P = (V l du) (x) e, 48 bytes per record instead of 64: that loop is bound
-    // by LDS bytes); the others send the four products.
-    constexpr bool COMPACT = sizeof(R) == 8 && !MF;
+    // The VALU FP64 one-bond-per-lane contraction sends V l du and reuses the
+    // record's unit vector (P = (V l du) (x) e, 48 bytes per record instead of
+    // 64, two more VALU per weight); the others send the four products, which
+    // is faster where the VALU is the bound (FP64 two bonds per lane: -3 %).
+    constexpr bool COMPACT = sizeof(R) == 8 && !MF && BPL == 1;
     R* rec = reinterpret_cast<R*>(__builtin_assume_aligned(rec64, 16));
     Geo<R> g[BPL];
 #pragma unroll

@@ -669,15 +669,18 @@ extern "C" int pd_update_positions(pd_context* c)
     return timed(c, 0, [&]() { return launch_positions(q, c->stream); });
 }
 
-static int force_and_velocity(pd_context* c, int write_a)
+// The force evaluation and velocity update of one step; with_next: the
+// gather also runs the next step's position update.
+static int force_and_velocity(pd_context* c, int write_a, bool with_next = false)
 {
     BondArgs a = bond_args(c);
     GatherArgs g = gather_args(c, write_a);
+    const PosArgs q = pos_args(c, true);
     int mixed = c->p.precision == PD_MIXED;
     const int dmg = c->dmgp.mode != PD_DAMAGE_OFF;
     if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->mf, dmg, c->n2_is_10, c->wpb, c->persist * c->n_cu, c->stream); }))
         return rc;
-    return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream); });
+    return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream, with_next ? &q : nullptr); });
 }
 
 // ---------------------------------------------------------------------------
@@ -801,11 +804,10 @@ extern "C" int pd_set_boundary(pd_context* c, const int* code, int n_groups, con
     return PD_OK;
 }
 
-static int enqueue_step(pd_context* c)
+static int enqueue_positions(pd_context* c)
 {
     PosArgs q = pos_args(c, true);
-    if (int rc = timed(c, 0, [&]() { return launch_positions(q, c->stream); })) return rc;
-    return force_and_velocity(c, 0);
+    return timed(c, 0, [&]() { return launch_positions(q, c->stream); });
 }
 
 extern "C" int pd_step(pd_context* c, int nsteps)
@@ -813,26 +815,38 @@ extern "C" int pd_step(pd_context* c, int nsteps)
     if (!c) return fail(nullptr, PD_ERR_INVALID, "ctx is NULL");
     if (nsteps < 0) return fail(c, PD_ERR_INVALID, "nsteps < 0");
     hipSetDevice(c->dev);
-    if (c->use_graph && !c->profile && nsteps > 1) {
-        if (!c->graph_valid) {
-            if (c->gexec) hipGraphExecDestroy(c->gexec);
-            if (c->graph) hipGraphDestroy(c->graph);
-            c->gexec = nullptr;
-            c->graph = nullptr;
-            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_step(c);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(c->stream, &g);
-            if (rc) return rc;
-            if (e != hipSuccess) return fail(c, PD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-            c->graph = g;
-            HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
-            c->graph_valid = true;
+    if (c->profile) {
+        // every phase on its own, so that each can be timed
+        for (int s = 0; s < nsteps; ++s) {
+            if (int rc = enqueue_positions(c)) return rc;
+            if (int rc = force_and_velocity(c, 0)) return rc;
         }
-        for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
-    } else {
-        for (int s = 0; s < nsteps; ++s)
-            if (int rc = enqueue_step(c)) return rc;
+    } else if (nsteps > 0) {
+        // positions, then (bond, gather + next positions) x (nsteps - 1), then
+        // bond and gather: the same operations as nsteps separate steps
+        if (int rc = enqueue_positions(c)) return rc;
+        if (c->use_graph && nsteps > 1) {
+            if (!c->graph_valid) {
+                if (c->gexec) hipGraphExecDestroy(c->gexec);
+                if (c->graph) hipGraphDestroy(c->graph);
+                c->gexec = nullptr;
+                c->graph = nullptr;
+                HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int rc = force_and_velocity(c, 0, true);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc) return rc;
+                if (e != hipSuccess) return fail(c, PD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+                c->graph = g;
+                HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+                c->graph_valid = true;
+            }
+            for (int s = 0; s + 1 < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            for (int s = 0; s + 1 < nsteps; ++s)
+                if (int rc = force_and_velocity(c, 0, true)) return rc;
+        }
+        if (int rc = force_and_velocity(c, 0)) return rc;
     }
     c->steps += nsteps;
     unsigned int first = 0xffffffffu;

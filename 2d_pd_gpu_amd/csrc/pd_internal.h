@@ -83,7 +83,9 @@ hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg,
 hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s);
 hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int wpb, hipStream_t s);
-hipError_t launch_gather(const GatherArgs& a, int mixed, hipStream_t s);
+// next != nullptr: also run the following step's position update (a
+// pd_step of several steps), as launch_positions(*next) would after it.
+hipError_t launch_gather(const GatherArgs& a, int mixed, hipStream_t s, const PosArgs* next = nullptr);
 hipError_t launch_positions(const PosArgs& a, hipStream_t s);
 hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double2* v,
                                    double2* a, hipStream_t s);

@@ -7,13 +7,35 @@
 
 namespace pd {
 
+// Velocity BC, a = f/rho, u += v dt + 1/2 a dt^2, fixed BC for particle i
+// (one step's prologue, src/main_seq.cpp:686-692), from its current v and f.
+// WRITE_V: store v even where no velocity BC applies (v was just updated).
+template <bool WRITE_V>
+__device__ __forceinline__ void position_update(const PosArgs& p, int i, double2 v, double2 f)
+{
+#pragma clang fp contract(off)
+    const int code = p.bc ? p.bc[i] : -1;
+    if (code >= 0) v = p.v0[code];
+    if (WRITE_V || code >= 0) p.v[i] = v;
+    const double2 acc = make_double2(f.x / p.rho, f.y / p.rho);
+    p.a[i] = acc;
+    double2 u = p.u[i];
+    const double dt2 = p.dt * p.dt;
+    u.x = u.x + (v.x * p.dt + (0.5 * acc.x) * dt2);
+    u.y = u.y + (v.y * p.dt + (0.5 * acc.y) * dt2);
+    if (code == -2) u = make_double2(0.0, 0.0);
+    p.u[i] = u;
+}
+
 // f_i = sum_j T_ij V_j - sum_j T_ji V_i ; v += 1/2 (a + f/rho) dt.
 // GATHER_LANES lanes per particle read its W row with unit stride (coalesced),
-// then reduce across the group.
+// then reduce across the group.  POS: the step is followed by another one in
+// the same pd_step call, so the next step's position update (k_positions)
+// runs here on the final f and v, with the same operations in the same order.
 constexpr int GATHER_LANES = 8;
 
-template <typename R>
-__global__ void __launch_bounds__(256) k_gather(GatherArgs g)
+template <typename R, bool POS>
+__global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
 {
     typedef typename Vec2<R>::T R2;
     const int blk = xcd_remap(blockIdx.x, gridDim.x, g.xcd);
@@ -47,34 +69,21 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs g)
         const double an0 = f0 / g.rho, an1 = f1 / g.rho;
         v.x = v.x + (0.5 * (a0.x + an0)) * g.dt;
         v.y = v.y + (0.5 * (a0.y + an1)) * g.dt;
-        g.v[i] = v;
         g.f[i] = make_double2(f0, f1);
-        if (g.write_a) g.a[i] = make_double2(an0, an1);
+        if constexpr (POS) {
+            position_update<true>(p, i, v, make_double2(f0, f1));
+        } else {
+            g.v[i] = v;
+            if (g.write_a) g.a[i] = make_double2(an0, an1);
+        }
     }
 }
 
-// Velocity BC, a = f/rho, u += v dt + 1/2 a dt^2, fixed BC (one step's
-// prologue, src/main_seq.cpp:686-692).
 __global__ void __launch_bounds__(256) k_positions(PosArgs p)
 {
-#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
-    const int code = p.bc ? p.bc[i] : -1;
-    double2 v = p.v[i];
-    if (code >= 0) {
-        v = p.v0[code];
-        p.v[i] = v;
-    }
-    const double2 f = p.f[i];
-    const double2 acc = make_double2(f.x / p.rho, f.y / p.rho);
-    p.a[i] = acc;
-    double2 u = p.u[i];
-    const double dt2 = p.dt * p.dt;
-    u.x = u.x + (v.x * p.dt + (0.5 * acc.x) * dt2);
-    u.y = u.y + (v.y * p.dt + (0.5 * acc.y) * dt2);
-    if (code == -2) u = make_double2(0.0, 0.0);
-    p.u[i] = u;
+    position_update<false>(p, i, p.v[i], p.f[i]);
 }
 
 __global__ void k_vbc_list(int count, const int* ids, double2 v0, double2* v, double2* a)
@@ -162,14 +171,22 @@ hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int w
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_gather(const GatherArgs& g, int mixed, hipStream_t s)
+hipError_t launch_gather(const GatherArgs& g, int mixed, hipStream_t s, const PosArgs* next)
 {
     const int grid = ceil_div((long)g.n * GATHER_LANES, 256);
     if (grid == 0) return hipSuccess;
-    if (mixed)
-        hipLaunchKernelGGL(k_gather<float>, dim3(grid), dim3(256), 0, s, g);
-    else
-        hipLaunchKernelGGL(k_gather<double>, dim3(grid), dim3(256), 0, s, g);
+    const PosArgs p = next ? *next : PosArgs{};
+    if (mixed) {
+        if (next)
+            hipLaunchKernelGGL((k_gather<float, true>), dim3(grid), dim3(256), 0, s, g, p);
+        else
+            hipLaunchKernelGGL((k_gather<float, false>), dim3(grid), dim3(256), 0, s, g, p);
+    } else {
+        if (next)
+            hipLaunchKernelGGL((k_gather<double, true>), dim3(grid), dim3(256), 0, s, g, p);
+        else
+            hipLaunchKernelGGL((k_gather<double, false>), dim3(grid), dim3(256), 0, s, g, p);
+    }
     return hipGetLastError();
 }
 
