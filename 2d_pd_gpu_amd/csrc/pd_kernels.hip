@@ -9,14 +9,14 @@ namespace pd {
 
 // Velocity BC, a = f/rho, u += v dt + 1/2 a dt^2, fixed BC for particle i
 // (one step's prologue, src/main_seq.cpp:686-692), from its current v and f.
-// WRITE_V: store v even where no velocity BC applies (v was just updated).
-template <bool WRITE_V>
 __device__ __forceinline__ void position_update(const PosArgs& p, int i, double2 v, double2 f)
 {
 #pragma clang fp contract(off)
     const int code = p.bc ? p.bc[i] : -1;
-    if (code >= 0) v = p.v0[code];
-    if (WRITE_V || code >= 0) p.v[i] = v;
+    if (code >= 0) {
+        v = p.v0[code];
+        p.v[i] = v;
+    }
     const double2 acc = make_double2(f.x / p.rho, f.y / p.rho);
     p.a[i] = acc;
     double2 u = p.u[i];
@@ -43,11 +43,23 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
     const int i = gid / GATHER_LANES;
     const int t = gid % GATHER_LANES;
     const bool ok = i < g.n;
+    const int ic = ok ? i : 0;
+    // the particle's own state is loaded before the row reduction (the lanes
+    // of a group share each address), so its latency overlaps the W reads
+    const int b0 = g.row[ic], b1 = g.row[ic + 1];
+    const double2 fo = g.fown[ic];
+    const double2 a0 = g.a[ic];
+    const double2 vi = g.v[ic];
+    double2 ui = make_double2(0.0, 0.0);
+    int code = -1;
+    if constexpr (POS) {
+        ui = p.u[ic];
+        code = p.bc ? p.bc[ic] : -1;
+    }
     const R2* W = reinterpret_cast<const R2*>(g.W);
     double f0 = 0.0, f1 = 0.0;
     if (ok) {
-        const int e = g.row[i + 1];
-        for (int b = g.row[i] + t; b < e; b += GATHER_LANES) {
+        for (int b = b0 + t; b < b1; b += GATHER_LANES) {
             const R2 w = W[b];
             f0 += double(w.x);
             f1 += double(w.y);
@@ -59,23 +71,30 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
         f1 += __shfl_xor(f1, off, 64);
     }
     if (!ok || t != 0) return;
-    const double2 fo = g.fown[i];
-    f0 = fo.x - f0;
-    f1 = fo.y - f1;
     {
 #pragma clang fp contract(off)
-        const double2 a0 = g.a[i];
-        double2 v = g.v[i];
-        const double an0 = f0 / g.rho, an1 = f1 / g.rho;
-        v.x = v.x + (0.5 * (a0.x + an0)) * g.dt;
-        v.y = v.y + (0.5 * (a0.y + an1)) * g.dt;
-        g.f[i] = make_double2(f0, f1);
-        if constexpr (POS) {
-            position_update<true>(p, i, v, make_double2(f0, f1));
-        } else {
-            g.v[i] = v;
-            if (g.write_a) g.a[i] = make_double2(an0, an1);
-        }
+    f0 = fo.x - f0;
+    f1 = fo.y - f1;
+    const double an0 = f0 / g.rho, an1 = f1 / g.rho;
+    double2 v;
+    v.x = vi.x + (0.5 * (a0.x + an0)) * g.dt;
+    v.y = vi.y + (0.5 * (a0.y + an1)) * g.dt;
+    g.f[i] = make_double2(f0, f1);
+    if constexpr (POS) {
+        // the next step's position update (k_positions), same operations
+        if (code >= 0) v = p.v0[code];
+        g.v[i] = v;
+        g.a[i] = make_double2(an0, an1);
+        const double dt2 = p.dt * p.dt;
+        double2 u;
+        u.x = ui.x + (v.x * p.dt + (0.5 * an0) * dt2);
+        u.y = ui.y + (v.y * p.dt + (0.5 * an1) * dt2);
+        if (code == -2) u = make_double2(0.0, 0.0);
+        p.u[i] = u;
+    } else {
+        g.v[i] = v;
+        if (g.write_a) g.a[i] = make_double2(an0, an1);
+    }
     }
 }
 
@@ -83,7 +102,7 @@ __global__ void __launch_bounds__(256) k_positions(PosArgs p)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
-    position_update<false>(p, i, p.v[i], p.f[i]);
+    position_update(p, i, p.v[i], p.f[i]);
 }
 
 __global__ void k_vbc_list(int count, const int* ids, double2 v0, double2* v, double2* a)
