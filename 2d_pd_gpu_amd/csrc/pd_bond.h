@@ -644,8 +644,12 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     // not for one-bond-per-lane 32-slot families (+7 % and more).
     // NC of the lane's BPL weight rows are kept for contraction 2, the others
     // are evaluated again there (FP64 two-bonds-per-lane: one row of 32 doubles
-    // fewer, measured -2.4 %)
-    constexpr int NC = (MF64 || (!MF && FS >= 64)) ? 0 : (sizeof(R) == 8 && BPL == 2) ? 1 : BPL;
+    // fewer, measured -2.4 %; three bonds per lane, rows of 48: one row in
+    // FP32, none in FP64)
+    constexpr int NC = (MF64 || (!MF && FS >= 64)) ? 0
+                       : BPL == 3                    ? (sizeof(R) == 8 ? 0 : 1)
+                       : (sizeof(R) == 8 && BPL == 2) ? 1
+                                                      : BPL;
     constexpr bool RW = NC < BPL;
     // neighbours per scheduling group of the contraction loops (each group's
     // record loads are issued together; 2 frees no registers in the FP64
@@ -875,9 +879,22 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             for (int l = l0; l < l0 + GL; ++l) {
                 if constexpr (MF32) {
                     const R Qa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
+                    R ex = R(0), ey = R(0), pe = R(0), qe = R(0);
+                    if constexpr (RW) {
+                        ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
+                        ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
+                        pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
+                        qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
+                    }
 #pragma unroll
-                    for (int s = 0; s < BPL; ++s)
-                        Sv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Qa, w[s][l], Sv[s], 0, 0, 0);
+                    for (int s = 0; s < BPL; ++s) {
+                        R om;
+                        if (s >= NC)
+                            om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        else
+                            om = w[s][l];
+                        Sv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Qa, om, Sv[s], 0, 0, 0);
+                    }
                 } else {
                     const R Q0 = rec[rec_at<R, FPW>(l, F.slot, 4)];
                     const R Q1 = rec[rec_at<R, FPW>(l, F.slot, 5)];

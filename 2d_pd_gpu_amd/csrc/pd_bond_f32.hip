@@ -1,5 +1,5 @@
 // pd_bond_f32.hip -- mixed-precision bond-kernel variants: tiles (32,1),
-// (64,1), (16,2), (32,2), each with VALU or matrix-core contractions, and
+// (64,1), (16,2), (32,2), (16,3), each with VALU or matrix-core contractions, and
 // with VALU contractions plus the damage update.
 #include "pd_bond_launch.h"
 
@@ -13,6 +13,7 @@ static hipError_t launch_f32_mf(const BondArgs& a, int lpf, int bpl, int n2_is_1
     if (bpl == 1 && lpf == 64) return launch_bond_n2<float, 64, 1, MF, DMG>(a, n2_is_10, wpb, pgrid, s);
     if (bpl == 2 && lpf == 16) return launch_bond_n2<float, 16, 2, MF, DMG>(a, n2_is_10, wpb, pgrid, s);
     if (bpl == 2 && lpf == 32) return launch_bond_n2<float, 32, 2, MF, DMG>(a, n2_is_10, wpb, pgrid, s);
+    if (bpl == 3 && lpf == 16) return launch_bond_n2<float, 16, 3, MF, DMG>(a, n2_is_10, wpb, pgrid, s);
     return hipErrorInvalidValue;
 }
 

@@ -1014,7 +1014,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     struct Var { int wpb, xcd, lpf, bpl, mf, persist; float ms; };
     std::vector<Var> vars;
     const int mixed = c->p.precision == PD_MIXED;
-    const int tiles[4][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}};
+    const int tiles[5][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}, {16, 3}};
     for (int mf : {0, 1})
         for (const auto& tl : tiles) {
             if (!tile_supported(mixed, tl[0], tl[1], mf) || tl[0] * tl[1] < c->maxM) continue;

@@ -67,8 +67,9 @@ struct PosArgs {
     double rho, dt;
 };
 
-// Tiles: (lanes per family, bonds per lane) in {(32,1), (64,1), (16,2)} and,
-// for mixed precision, (32,2); a family may hold up to lpf * bpl bonds.
+// Tiles: (lanes per family, bonds per lane) in {(32,1), (64,1), (16,2),
+// (16,3)} and, for mixed precision, (32,2); a family may hold up to
+// lpf * bpl bonds.
 // mf = 1: family contractions on the matrix cores (every mixed tile; FP64
 // tiles (32,1) and (64,1)).
 bool tile_supported(int mixed, int lpf, int bpl, int mf);

@@ -159,6 +159,7 @@ bool tile_supported(int mixed, int lpf, int bpl, int mf)
     if (mf && !mixed) return bpl == 1 && (lpf == 32 || lpf == 64);
     if (bpl == 1) return lpf == 32 || lpf == 64;
     if (bpl == 2) return lpf == 16 || (lpf == 32 && mixed);
+    if (bpl == 3) return lpf == 16;
     return false;
 }
 
@@ -187,6 +188,7 @@ hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int w
     if (bpl == 1 && lpf == 64) return n2_is_10 ? launch_shape_t<64, 1, 1>(a, wpb, s) : launch_shape_t<64, 1, 0>(a, wpb, s);
     if (bpl == 2 && lpf == 16) return n2_is_10 ? launch_shape_t<16, 2, 1>(a, wpb, s) : launch_shape_t<16, 2, 0>(a, wpb, s);
     if (bpl == 2 && lpf == 32) return n2_is_10 ? launch_shape_t<32, 2, 1>(a, wpb, s) : launch_shape_t<32, 2, 0>(a, wpb, s);
+    if (bpl == 3 && lpf == 16) return n2_is_10 ? launch_shape_t<16, 3, 1>(a, wpb, s) : launch_shape_t<16, 3, 0>(a, wpb, s);
     return hipErrorInvalidValue;
 }
 

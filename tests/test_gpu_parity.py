@@ -268,7 +268,7 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     pdh = pdpkg.pdhip
     ref, bref, _ = run_plate_gpu(pdh, small_plate, 0.005, 40)
     mask = small_plate.code >= 0
-    for tile in (3201, 6401, 1602, 13201, 16401):
+    for tile in (3201, 6401, 1602, 1603, 13201, 16401):
         tile_ref = None
         for wpb, xcd, persist in ((1, 0, 0), (2, 1, 0), (8, 0, 0), (8, 1, 0), (4, 1, 1), (2, 0, 3),
                                   (8, 1, 2)):
@@ -291,7 +291,7 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
         _, nonbc, _ = H.parity_norms(tile_ref, ref["v"], mask)
         assert nonbc < 1e-13
     mref = None
-    for tile in (3201, 6401, 1602, 3202, 13201, 16401, 11602, 13202):
+    for tile in (3201, 6401, 1602, 3202, 1603, 13201, 16401, 11602, 13202, 11603):
         ctx = small_plate.context(precision=1, s0=0.005)
         ctx.set_option(pdh.OPT_TILE, tile)
         ctx.step(40)
@@ -359,7 +359,7 @@ def test_plate_matrix_core_breaking_matches_oracle(pdpkg, small_plate, small_pla
     assert nonbc < 1e-5
 
 
-@pytest.mark.parametrize("precision,tile", [(0, 0), (1, 0), (0, 13201), (1, 13201)])
+@pytest.mark.parametrize("precision,tile", [(0, 0), (1, 0), (0, 13201), (1, 13201), (0, 1603), (1, 11603)])
 def test_persistent_bond_kernel_bitwise(pdpkg, precision, tile):
     """The persistent bond kernel (each wave walks many family groups and
     prefetches the next one) gives the same bits as one wave per group."""
@@ -393,7 +393,7 @@ def test_per_particle_youngs_modulus(pdpkg):
 
 
 def test_delta_4015_family_of_48(pdpkg):
-    say("delta = 4.015 dx, families of 48, 64 lanes per family")
+    say("delta = 4.015 dx, families of 48: 64 lanes per family, or 16 lanes x 3 bonds")
     w = pdpkg.plates.single_plate("d4", 24, 22, 20.0, horizon=4.015, seed=5, porosity=0.01)
     assert np.diff(w.row).max() == 48
     u, v, f, br = run_plate_oracle(w, 0.0, 30)
@@ -401,10 +401,11 @@ def test_delta_4015_family_of_48(pdpkg):
     assert stats.lanes_per_family * stats.bonds_per_lane >= 48
     _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
     assert nonbc < 1e-11
-    st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, tile=16401)
-    _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
-    assert nonbc < 1e-11
-    for tile in (6401, 16401):
+    for tile in (16401, 1603):
+        st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, tile=tile)
+        _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
+        assert nonbc < 1e-11
+    for tile in (6401, 16401, 1603, 11603):
         st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, precision=1, tile=tile)
         _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
         assert nonbc < 1e-5
