@@ -1021,7 +1021,10 @@ extern "C" int pd_autotune(pd_context* c, int steps)
             for (int w : {1, 2, 4, 8})
                 for (int x : {1, 8})
                     for (int pz : {0, 1, 2, 4})
-                        if (pz == 0 || w * pz <= 32) vars.push_back({w, x, tl[0], tl[1], mf, pz, 0.f});
+                        // a persistent grid needs at least two resident waves per SIMD
+                        // to hide its loads (fewer measured 2-6x slower)
+                        if (pz == 0 || (w * pz >= 8 && w * pz <= 32))
+                            vars.push_back({w, x, tl[0], tl[1], mf, pz, 0.f});
         }
     const Var keep = {c->wpb, c->xcd, c->lpf, c->bpl, c->mf, c->persist, 0.f};
     auto use = [&](const Var& v) {

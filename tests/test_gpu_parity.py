@@ -199,13 +199,14 @@ def run_plate_oracle(w, s0, steps, E=None):
     return u, v, f, br
 
 
-def run_plate_gpu(pdh, w, s0, steps, precision=0, E=None, chunks=1, graph=True, tile=None):
+def run_plate_gpu(pdh, w, s0, steps, precision=0, E=None, chunks=1, graph=True, tile=None, profile=False):
     ctx = w.context(precision=precision, s0=s0)
     if tile is not None:
         ctx.set_option(pdh.OPT_TILE, tile)
     if E is not None:
         ctx.set_youngs_modulus(E)
     ctx.set_option(pdh.OPT_USE_GRAPH, int(graph))
+    ctx.set_option(pdh.OPT_PROFILE, int(profile))
     for _ in range(chunks):
         ctx.step(steps // chunks)
     st = ctx.get_state()
@@ -250,17 +251,22 @@ def test_plate_mixed_breaking_vs_oracle(pdpkg, small_plate, small_plate_oracle):
     np.testing.assert_array_equal(brg, br)
 
 
-def test_plate_graph_and_chunks_bitwise(pdpkg, small_plate):
-    say("graph replay vs eager, chunked steps")
+@pytest.mark.parametrize("precision", [0, 1])
+def test_plate_graph_and_chunks_bitwise(pdpkg, small_plate, precision):
+    """Graph replay, eager launches, chunked pd_step calls and the per-phase
+    profiling path (no fused gather + position update) give the same bits."""
+    say(f"graph replay vs eager vs per-phase, chunked steps, precision {precision}")
     pdh = pdpkg.pdhip
-    a, ba, _ = run_plate_gpu(pdh, small_plate, 0.005, 60, graph=True)
-    b, bb, _ = run_plate_gpu(pdh, small_plate, 0.005, 60, graph=False)
-    c, bc, _ = run_plate_gpu(pdh, small_plate, 0.005, 60, graph=True, chunks=3)
-    for k in ("u", "v", "f"):
-        np.testing.assert_array_equal(a[k], b[k])
-        np.testing.assert_array_equal(a[k], c[k])
-    np.testing.assert_array_equal(ba, bb)
-    np.testing.assert_array_equal(ba, bc)
+    runs = [run_plate_gpu(pdh, small_plate, 0.005, 60, precision=precision, graph=True),
+            run_plate_gpu(pdh, small_plate, 0.005, 60, precision=precision, graph=False),
+            run_plate_gpu(pdh, small_plate, 0.005, 60, precision=precision, graph=True, chunks=3),
+            run_plate_gpu(pdh, small_plate, 0.005, 60, precision=precision, profile=True)]
+    a, ba, _ = runs[0]
+    assert ba.sum() > small_plate.broken.sum()  # bonds broke during the run
+    for b, bb, _ in runs[1:]:
+        for k in ("u", "v", "a", "f"):
+            np.testing.assert_array_equal(a[k], b[k])
+        np.testing.assert_array_equal(ba, bb)
 
 
 def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
