@@ -146,9 +146,9 @@ def main():
                     help="bytes of the plain copy probe (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: the step shards by space across GPUs "
-              f"(see DESIGN.md), but no multi-GPU path is built; got --gpus {args.gpus}",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only: the step would shard by space across GPUs "
+              f"(Morton slabs with a 2-horizon halo), but no multi-GPU path is built; "
+              f"got --gpus {args.gpus}", file=sys.stderr)
         sys.exit(2)
     pkg = importlib.import_module("2d_pd_gpu_amd")
     t = time.perf_counter()
