@@ -43,6 +43,20 @@ def kernel_bytes(n, nb, mixed):
     return bond, step
 
 
+def phase_bytes(n, nb, mixed):
+    """Algorithmic bytes of each kernel phase as timed by the events (eager,
+    unfused): the bond kernel as above; the gather reads row_ptr 4, f_own 16,
+    a 16, v 16 and writes v 16, f 16 per particle and reads its W row (16 FP64 /
+    8 FP32 per bond); the position update reads v, f, u 16 each and the BC code
+    4, and writes a and u 16 each."""
+    bond, _ = kernel_bytes(n, nb, mixed)
+    return {"bond": bond, "gather": 84 * n + (8 if mixed else 16) * nb, "positions": 84 * n}
+
+
+def phase_fracs(pb, ms):
+    return {k: pb[k] / (ms[k] * 1e-3) / (HBM_PEAK_GBS * 1e9) if ms[k] > 0 else None for k in pb}
+
+
 def cpu_baseline(pkg, w, state, budget_s=12.0):
     """Literal long-double restatement of computeVelocity, 1 thread, on the
     first families of the plate (in caller order) until ~budget_s of work."""
@@ -208,6 +222,8 @@ def main():
         "step_hbm_frac": r["sb"] / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "copy_probe_gbs": copy_gbs,
         "kernel_ms": {"bond": r["bond_ms"], "gather": r["gather_ms"], "positions": r["pos_ms"]},
+        "phase_hbm_frac": phase_fracs(phase_bytes(w.n, w.nb, False),
+                                      {"bond": r["bond_ms"], "gather": r["gather_ms"], "positions": r["pos_ms"]}),
         "tile": tile_info(r["tile"]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "kernel": "k_bond<double>",
@@ -228,6 +244,8 @@ def main():
             "particle_steps_per_s": w.n * args.steps / m["t"],
             "step_hbm_frac": m["sb"] / (ms_m * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "kernel_ms": {"bond": m["bond_ms"], "gather": m["gather_ms"], "positions": m["pos_ms"]},
+            "phase_hbm_frac": phase_fracs(phase_bytes(w.n, w.nb, True),
+                                          {"bond": m["bond_ms"], "gather": m["gather_ms"], "positions": m["pos_ms"]}),
             "tile": tile_info(m["tile"]),
             "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach_m / HBM_PEAK_GBS, "kernel": "k_bond<float>",
