@@ -1,8 +1,8 @@
 """The benchmark configurations at their full sizes.
 
-C1 (100x100 plate, 1000 steps) is small enough for the FP64 oracle.  At the
-larger sizes the oracle is too slow, so the tests use properties that hold at
-any size:
+C1 (100x100 plate, 1000 steps) is small enough for the FP64 oracle, and so is
+one step of C2 (1024 x 1024).  For longer runs at the larger sizes the oracle
+is too slow, so the tests use properties that hold at any size:
 * determinism of the graph-replayed step;
 * linear momentum balance sum_i V_i f_i = 0 (pairwise antisymmetric forces);
 * symmetry of the broken-bond set;
@@ -38,6 +38,28 @@ def test_c1_plate_100_1000_steps_matches_oracle(pdpkg):
         _, nonbc, _ = H.parity_norms(st["v"], v, mask)
         assert nonbc < tol, (prec, nonbc)
         assert np.linalg.norm(st["u"] - u) <= tol * np.linalg.norm(u)
+
+
+def test_c2_plate_1024_step_1_matches_oracle(pdpkg):
+    """The C2 check of the configuration: step-1 velocities of the full plate
+    against the FP64 oracle, within 1e-12 (non-prescribed particles)."""
+    say("C2: plate_1024, step 1, FP64 vs the FP64 oracle (16 threads)")
+    w = pdpkg.plates.plate_1024()
+    p = O.or_params(horizon=w.horizon, dx=w.dx, rho=8000.0, dt=1e-8, E=192e9, nu=0.3, s0=0.02)
+    rev = O.build_rev(w.row, w.col)
+    u, v, a, f = (np.zeros(2 * w.n) for _ in range(4))
+    br = w.broken.copy()
+    O.step(p, w.X, w.V, w.row, w.col, rev, w.code, w.v0, u, v, a, f, br, 1, nthreads=16)
+    ctx = w.context(precision=0)
+    ctx.step(1)
+    st = ctx.get_state()
+    brg = ctx.get_broken()
+    ctx.close()
+    np.testing.assert_array_equal(brg, br)
+    np.testing.assert_array_equal(st["u"], u)
+    _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
+    assert nonbc < 1e-12, nonbc
+    assert np.linalg.norm(st["f"] - f) <= 1e-12 * np.linalg.norm(f)
 
 
 def test_c2_plate_1024_full_size_properties(pdpkg):
