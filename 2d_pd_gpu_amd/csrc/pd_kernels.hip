@@ -33,6 +33,7 @@ __device__ __forceinline__ void position_update(const PosArgs& p, int i, double2
 // the same pd_step call, so the next step's position update (k_positions)
 // runs here on the final f and v, with the same operations in the same order.
 constexpr int GATHER_LANES = 8;
+constexpr int GATHER_UNROLL = 4;
 
 template <typename R, bool POS>
 __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
@@ -58,8 +59,24 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
     }
     const R2* W = reinterpret_cast<const R2*>(g.W);
     double f0 = 0.0, f1 = 0.0;
+    // the first GATHER_UNROLL entries of the lane's share of the row are loaded
+    // together (indices clamped to entry 0, values masked), so they are in
+    // flight at once; longer rows continue in the loop.  Same summation order.
+    R2 wv[GATHER_UNROLL];
+#pragma unroll
+    for (int q = 0; q < GATHER_UNROLL; ++q) {
+        const int b = b0 + t + q * GATHER_LANES;
+        wv[q] = W[ok && b < b1 ? b : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < GATHER_UNROLL; ++q) {
+        if (ok && b0 + t + q * GATHER_LANES < b1) {
+            f0 += double(wv[q].x);
+            f1 += double(wv[q].y);
+        }
+    }
     if (ok) {
-        for (int b = b0 + t; b < b1; b += GATHER_LANES) {
+        for (int b = b0 + t + GATHER_UNROLL * GATHER_LANES; b < b1; b += GATHER_LANES) {
             const R2 w = W[b];
             f0 += double(w.x);
             f1 += double(w.y);
