@@ -106,3 +106,26 @@ def test_c3_plate_2048_per_particle_E_mixed_vs_fp64(pdpkg):
     for k in ("u", "f"):
         assert np.linalg.norm(m[k] - a[k]) <= 1e-4 * np.linalg.norm(a[k])
     np.testing.assert_array_equal(bm, ba)
+
+
+@pytest.mark.parametrize("name,steps", [("plate_1024", 5000), ("plate_2048", 2000)])
+def test_full_length_runs_mixed_vs_fp64(pdpkg, name, steps):
+    """C2 and C3 at their configured lengths: every pd_step checks the state
+    for non-finite values; mixed precision keeps the FP64 broken-bond set and
+    stays within 1e-5 of the FP64 velocities; forces stay pair-balanced."""
+    say(f"{name}: {steps} steps, FP64 and mixed")
+    w = getattr(pdpkg.plates, name)()
+    out = []
+    for prec in (0, 1):
+        ctx = w.context(precision=prec)
+        for _ in range(steps // 500):
+            ctx.step(500)
+        out.append((ctx.get_state(), ctx.get_broken()))
+        ctx.close()
+    (a, ba), (m, bm) = out
+    np.testing.assert_array_equal(bm, ba)
+    np.testing.assert_array_equal(ba, ba[O.build_rev(w.row, w.col)])
+    _, nonbc, _ = H.parity_norms(m["v"], a["v"], w.code >= 0)
+    assert nonbc < 1e-5, nonbc
+    Vf = a["f"].reshape(-1, 2) * w.V[:, None]
+    assert np.all(np.abs(Vf.sum(axis=0)) <= 1e-12 * np.abs(Vf).sum(axis=0))
