@@ -1,6 +1,6 @@
 // pd_bond_f32.hip -- mixed-precision bond-kernel variants: tiles (32,1),
-// (64,1), (16,2), (32,2), (16,3), each with VALU or matrix-core contractions, and
-// with VALU contractions plus the damage update.
+// (64,1), (16,2), (32,2), (16,3), each with VALU or matrix-core contractions,
+// with and without the damage update.
 #include "pd_bond_launch.h"
 
 namespace pd {
@@ -20,7 +20,8 @@ static hipError_t launch_f32_mf(const BondArgs& a, int lpf, int bpl, int n2_is_1
 hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s)
 {
-    if (dmg) return launch_f32_mf<0, 1>(a, lpf, bpl, n2_is_10, wpb, pgrid, s);
+    if (dmg) return mf ? launch_f32_mf<1, 1>(a, lpf, bpl, n2_is_10, wpb, pgrid, s)
+                       : launch_f32_mf<0, 1>(a, lpf, bpl, n2_is_10, wpb, pgrid, s);
     return mf ? launch_f32_mf<1, 0>(a, lpf, bpl, n2_is_10, wpb, pgrid, s)
               : launch_f32_mf<0, 0>(a, lpf, bpl, n2_is_10, wpb, pgrid, s);
 }

@@ -1,6 +1,6 @@
 // pd_bond_f64.hip -- FP64 bond-kernel variants: tiles (32,1), (64,1), (16,2),
-// (16,3) with VALU contractions (with and without the damage update), (32,1) and
-// (64,1) with matrix-core contractions.
+// (16,3) with VALU contractions, (32,1) and (64,1) with matrix-core
+// contractions, each with and without the damage update.
 #include "pd_bond_launch.h"
 
 namespace pd {
@@ -8,7 +8,10 @@ namespace pd {
 hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s)
 {
-    if (dmg) {
+    if (dmg && mf) {
+        if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1, 1, 1>(a, n2_is_10, wpb, pgrid, s);
+        if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1, 1, 1>(a, n2_is_10, wpb, pgrid, s);
+    } else if (dmg) {
         if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1, 0, 1>(a, n2_is_10, wpb, pgrid, s);
         if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1, 0, 1>(a, n2_is_10, wpb, pgrid, s);
         if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2, 0, 1>(a, n2_is_10, wpb, pgrid, s);

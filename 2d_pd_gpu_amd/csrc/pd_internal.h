@@ -75,8 +75,7 @@ struct PosArgs {
 bool tile_supported(int mixed, int lpf, int bpl, int mf);
 // pgrid > 0: persistent launch with pgrid workgroups (each walks a contiguous
 // range of family groups, prefetching the next group's loads).
-// dmg = 1: also update the Mazars bond damage (VALU contractions only; a
-// matrix-core tile runs its VALU twin).
+// dmg = 1: also update the Mazars bond damage.
 hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int mf, int dmg, int n2_is_10,
                        int wpb, int pgrid, hipStream_t s);
 hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,

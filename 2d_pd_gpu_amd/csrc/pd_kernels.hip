@@ -184,7 +184,6 @@ hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int mf, i
                        int wpb, int pgrid, hipStream_t s)
 {
     if (!tile_supported(mixed, lpf, bpl, mf)) return hipErrorInvalidValue;
-    if (dmg) mf = 0;
     return mixed ? launch_bond_f32(a, lpf, bpl, mf, dmg, n2_is_10, wpb, pgrid, s)
                  : launch_bond_f64(a, lpf, bpl, mf, dmg, n2_is_10, wpb, pgrid, s);
 }

@@ -110,6 +110,40 @@ def test_damage_during_steps_matches_oracle(pdpkg, plate, mode):
         assert agree.mean() > 0.99
 
 
+@pytest.mark.parametrize("precision,tile", [(0, 13201), (0, 16401), (1, 11602), (1, 13201)])
+def test_damage_with_matrix_core_contractions(pdpkg, plate, precision, tile):
+    """The fused damage update on the matrix-core tiles, 300 steps against the
+    oracle (mode 2, the published law)."""
+    say(f"damage on matrix-core tile {tile}, precision {precision}, 300 steps")
+    pdh = pdpkg.pdhip
+    w = plate
+    p = oracle_params(w)
+    p.s0 = 0.005
+    rev = O.build_rev(w.row, w.col)
+    u, v, a, f = (np.zeros(2 * w.n) for _ in range(4))
+    br = w.broken.copy()
+    kap = np.full(w.nb, KAPPA0)
+    dm = np.zeros(w.nb)
+    st = np.zeros(w.n)
+    dp = O.damage_params(mode=2, kappa0=KAPPA0)
+    O.step_damage(p, dp, w.X, w.V, w.row, w.col, rev, w.code, w.v0, u, v, a, f, br, kap, dm, st, 300)
+    ctx = w.context(precision=precision, s0=0.005)
+    ctx.set_option(pdh.OPT_TILE, tile)
+    ctx.set_damage(pdh.DamageParams.make(mode=2, kappa0=KAPPA0))
+    ctx.step(300)
+    g = ctx.get_damage()
+    stats = ctx.stats()
+    brg = ctx.get_broken()
+    ctx.close()
+    assert stats.matrix_cores == 1
+    np.testing.assert_array_equal(brg, br)
+    tol = 1e-8 if precision == 0 else 1e-3
+    assert np.linalg.norm(g["kappa"] - kap) <= tol * np.linalg.norm(kap)
+    assert np.linalg.norm(g["status"] - st) <= tol * np.linalg.norm(st)
+    if precision == 0:
+        assert np.linalg.norm(g["bond"] - dm) <= tol * np.linalg.norm(dm)
+
+
 def test_damage_persistent_and_off(pdpkg, plate):
     say("damage: persistent kernel bitwise, mode off stops updates")
     pdh = pdpkg.pdhip
