@@ -415,6 +415,17 @@ def test_delta_4015_family_of_48(pdpkg):
         st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, precision=1, tile=tile)
         _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
         assert nonbc < 1e-5
+    # K^-1 assembled again by the (16, 3) shape kernel (pd_set_broken with the
+    # tile set) gives the same step
+    pdh = pdpkg.pdhip
+    for prec, tol in ((0, 1e-11), (1, 1e-5)):
+        ctx = w.context(precision=prec, s0=0.0)
+        ctx.set_option(pdh.OPT_TILE, 1603)
+        ctx.set_broken(ctx.get_broken())
+        ctx.step(30)
+        _, nonbc, _ = H.parity_norms(ctx.get_state()["v"], v, w.code >= 0)
+        ctx.close()
+        assert nonbc < tol
 
 
 def test_batched_plates_equal_individual_runs(pdpkg):
