@@ -126,6 +126,14 @@ __device__ __forceinline__ R weight(R hxk, R hyk, R pk, R qk, R exl, R eyl, R pl
 
 // Sum over the lanes of one family (LPF lanes, aligned).
 template <int LPF>
+__device__ __forceinline__ float family_sum(float x)
+{
+#pragma unroll
+    for (int off = LPF / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+template <int LPF>
 __device__ __forceinline__ double family_sum(double x)
 {
 #pragma unroll
@@ -931,9 +939,11 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             for (int c = 0; c < 4; ++c) Sm[s][c] = Sv[s][c];
     }
     }
-    double vsum = 0.0;
+    // V_H and the own-bond force sum accumulate in the bond arithmetic's
+    // precision (FP32 in mixed precision, like the T they sum; measured -2 %)
+    R vsum = R(0);
 #pragma unroll
-    for (int s = 0; s < BPL; ++s) vsum += alive[s] ? S[s].Vj : 0.0;
+    for (int s = 0; s < BPL; ++s) vsum += alive[s] ? R(S[s].Vj) : R(0);
     const double VH = family_sum<LPF>(vsum);
     // 1 / V_H: FP64 division, or the hardware reciprocal in FP32
     R ivh;
@@ -942,7 +952,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     else
         ivh = VH > 0.0 ? __builtin_amdgcn_rcpf(float(VH)) : 0.0f;
     R T0[BPL], T1[BPL];
-    double fo0 = 0.0, fo1 = 0.0;
+    R fo0 = R(0), fo1 = R(0);
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         T0[s] = R(0);
@@ -952,14 +962,14 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             T0[s] = (Sm[s][0] * x0r + Sm[s][1] * x1r) * ivh;
             T1[s] = (Sm[s][2] * x0r + Sm[s][3] * x1r) * ivh;
         }
-        fo0 += double(T0[s]) * S[s].Vj;
-        fo1 += double(T1[s]) * S[s].Vj;
+        fo0 += T0[s] * R(S[s].Vj);
+        fo1 += T1[s] * R(S[s].Vj);
     }
 
     // ---- outputs: own-bond sum and the reverse-slot terms ----
     fo0 = family_sum<LPF>(fo0);
     fo1 = family_sum<LPF>(fo1);
-    if (F.famok && F.t == 0) a.fown[F.fam] = make_double2(fo0, fo1);
+    if (F.famok && F.t == 0) a.fown[F.fam] = make_double2(double(fo0), double(fo1));
     if constexpr (DMG) {
         // damageStatus_i = sum_k d_ik / |H_i| (src/main_seq.cpp:593)
         dsum = family_sum<LPF>(dsum);
