@@ -428,6 +428,70 @@ def test_delta_4015_family_of_48(pdpkg):
         assert nonbc < tol
 
 
+def test_families_of_60_in_64_slot_tiles(pdpkg):
+    """The largest families the tiles take: delta = 4.4 dx gives 60 bonds per
+    interior family (64-slot tiles) against the oracle, with bonds breaking.
+
+    FP64 runs 200 steps, until 7784 of the 21872 bonds are broken.  Mixed
+    precision is checked after 50 steps (1594 broken, largest condition number
+    of an alive bond's K 8.4e3).  From about step 100 on, a few bonds are left
+    with nearly collinear neighbours and K condition numbers up to 4.3e9,
+    which FP32 products cannot resolve (their error grows as 6e-8 x cond), so
+    mixed precision leaves the 1e-5 bound there by design."""
+    say("delta = 4.4 dx, families of 60, 64-slot tiles")
+    w = pdpkg.plates.single_plate("h44", 22, 20, 30.0, horizon=4.4, seed=6, notch_len_cells=8)
+    assert np.diff(w.row).max() == 60
+    mask = w.code >= 0
+    for prec, steps, tiles, tol in ((0, 200, (6401, 16401), 1e-10),
+                                    (1, 50, (6401, 16401, 3202, 13202), 1e-5)):
+        u, v, f, br = run_plate_oracle(w, 0.004, steps)
+        assert br.sum() > w.broken.sum() + 500   # 898 broken by the notch
+        for tile in tiles:
+            st, brg, _ = run_plate_gpu(pdpkg.pdhip, w, 0.004, steps, precision=prec, tile=tile)
+            np.testing.assert_array_equal(brg, br)
+            _, nonbc, _ = H.parity_norms(st["v"], v, mask)
+            assert nonbc < tol, (prec, tile, nonbc)
+
+
+def test_isolated_particles_and_tiny_families(pdpkg):
+    """Ragged input next to a plate: particles with no family, a two-particle
+    body (one bond each way: K is rank one, so singular, and carries no force)
+    and a three-particle body (families of two bonds, one corner driven); every
+    tile against the oracle."""
+    say("isolated particles, one- and two-bond families")
+    pl = pdpkg.plates
+    base = pl.single_plate("rag", 20, 16, 20.0, notch_len_cells=6, seed=9)
+    dx = base.dx
+    far = 40 * dx
+    extra = np.array([far, far, far + 10 * dx, far, far, far + 10 * dx,            # isolated
+                      -far, -far, -far + 1.5 * dx, -far,                            # pair
+                      far, -far, far + 1.5 * dx, -far, far, -far + 1.5 * dx])       # triangle
+    X = np.concatenate([base.X, extra])
+    n = X.size // 2
+    V = np.full(n, base.V[0])
+    row, col = pdpkg.pdhip.build_families(X, dx, base.horizon)
+    lens = np.diff(row)[base.n:]
+    np.testing.assert_array_equal(lens, [0, 0, 0, 1, 1, 2, 2, 2])
+    code = np.concatenate([base.code, np.array([-1, -1, -1, 0, -1, 0, -1, -1], dtype=np.int32)])
+    broken = np.zeros(len(col), dtype=np.uint8)
+    broken[:len(base.col)] = base.broken  # the plate's bonds come first (same families)
+    w = pl.Workload("rag+", X, V, row, col, code, base.v0, broken, horizon=base.horizon, dx=dx)
+    u, v, f, br = run_plate_oracle(w, 0.0, 40)
+    fx = f.reshape(-1, 2)[base.n:]
+    assert np.all(fx[:5] == 0.0) and np.abs(fx[5:]).max() > 0
+    mask = w.code >= 0
+    for prec, tiles, tol in ((0, (3201, 6401, 1602, 1603, 13201, 16401), 1e-11),
+                             (1, (3201, 1602, 3202, 11602, 1603), 1e-5)):
+        for tile in tiles:
+            st, _, stats = run_plate_gpu(pdpkg.pdhip, w, 0.0, 40, precision=prec, tile=tile)
+            _, nonbc, _ = H.parity_norms(st["v"], v, mask)
+            assert nonbc < tol, (prec, tile, nonbc)
+            g = st["f"].reshape(-1, 2)[base.n:]
+            assert np.all(g[:5] == 0.0)                 # no family, or a singular one
+            assert np.abs(g[5:]).max() > 0
+            assert stats.n_singular >= 2                # the pair's two bonds
+
+
 def test_batched_plates_equal_individual_runs(pdpkg):
     say("batched plates (disjoint union) vs each plate alone")
     pdh = pdpkg.pdhip
