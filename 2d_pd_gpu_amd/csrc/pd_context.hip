@@ -939,6 +939,12 @@ extern "C" int pd_get_stats(pd_context* c, pd_stats* s)
         long nbk = 0;
         for (unsigned char x : t) nbk += x;
         s->n_broken = nbk;
+        unsigned int nill = 0;
+        HIPCHK(c, launch_count_ill_conditioned(c->nb, c->kinv, c->p.precision == PD_MIXED, c->broken,
+                                               PD_ILL_CONDITIONED, c->counters + 3, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&nill, c->counters + 3, sizeof(nill), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        s->n_ill_conditioned = nill;
     }
     for (int k = 0; k < 4; ++k) {
         s->phase_ms[k] = c->phase_ms[k];

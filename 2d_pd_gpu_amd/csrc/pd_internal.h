@@ -34,7 +34,8 @@ struct BondArgs {
     double brk_c;             // (1 + s0)^2
     double ktol;              // singular-K tolerance
     int force_k;              // 1: assemble K for every family in this launch
-    unsigned int* counters;   // [0] newly broken bonds, [1] singular K, [2] first non-finite
+    unsigned int* counters;   // [0] newly broken bonds, [1] singular K, [2] first non-finite,
+                              // [3] ill-conditioned K (pd_get_stats)
     // Mazars bond damage (kernels built with damage only)
     int dmg_mode;             // 1: running threshold in the law (reference), 2: initial
     double kap0, At, Bt, Ac, Bc;
@@ -91,6 +92,8 @@ hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double
                                    double2* a, hipStream_t s);
 hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
+hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32, const unsigned char* broken,
+                                       double cmin, unsigned int* count, hipStream_t s);
 // *first = smallest index with a non-finite u, v or f (0xffffffff if none).
 hipError_t launch_check_finite(int n, const double2* u, const double2* v, const double2* f,
                                unsigned int* first, hipStream_t s);

@@ -151,6 +151,31 @@ __global__ void __launch_bounds__(256) k_check_finite(int n, const double2* __re
         atomicMin(first, (unsigned int)i);
 }
 
+// Alive, non-singular bonds whose shape tensor has condition number >= cmin,
+// read from the stored K^-1 (same eigenvectors, reciprocal eigenvalues).  For
+// a symmetric 2x2 matrix with eigenvalues l1 >= l2 > 0 and c = l1 / l2,
+// tr^2 / det = c + 2 + 1/c, so c >= cmin exactly when tr^2 >= r det with
+// r = cmin + 2 + 1/cmin.  A stored inverse whose det is not positive (an FP32
+// rounding of a nearly singular one) counts as ill-conditioned.  Broken and
+// singular bonds store K^-1 = 0 and are skipped.
+template <typename KT>
+__global__ void __launch_bounds__(256) k_count_ill_conditioned(long nb, const KT* __restrict__ kinv,
+                                                               const unsigned char* __restrict__ broken,
+                                                               double r, unsigned int* __restrict__ count)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    unsigned int mine = 0;
+    for (long b = (long)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += stride) {
+        const double a = kinv[3 * b], o = kinv[3 * b + 1], c = kinv[3 * b + 2];
+        if (broken[b] || (a == 0.0 && o == 0.0 && c == 0.0)) continue;
+        const double det = a * c - o * o, tr = a + c;
+        mine += (det <= 0.0 || tr * tr >= r * det) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off, 64);
+    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(count, mine);
+}
+
 __global__ void k_fill_u32(unsigned int* p, unsigned int v, int count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,6 +281,23 @@ hipError_t launch_check_finite(int n, const double2* u, const double2* v, const 
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, first, 0xffffffffu, 1);
     hipLaunchKernelGGL(k_check_finite, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, u, v, f, first);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32, const unsigned char* broken,
+                                       double cmin, unsigned int* count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, count, 0u, 1);
+    if (nb <= 0) return hipGetLastError();
+    const long want = (nb + 255) / 256;
+    const int grid = (int)(want < 256L * 16 ? want : 256L * 16);
+    const double r = cmin + 2.0 + 1.0 / cmin;
+    if (kinv_f32)
+        hipLaunchKernelGGL(k_count_ill_conditioned<float>, dim3(grid), dim3(256), 0, s, nb,
+                           static_cast<const float*>(kinv), broken, r, count);
+    else
+        hipLaunchKernelGGL(k_count_ill_conditioned<double>, dim3(grid), dim3(256), 0, s, nb,
+                           static_cast<const double*>(kinv), broken, r, count);
     return hipGetLastError();
 }
 

@@ -120,7 +120,16 @@ typedef struct pd_stats {
     double phase_ms[4];    /* profiled time: positions, bonds, gather, other */
     long   phase_calls[4];
     int    matrix_cores;   /* 1: family contractions on the matrix cores (MFMA) */
+    long   n_ill_conditioned; /* alive bonds whose K has condition number >= PD_ILL_CONDITIONED
+                                 now (counted from the stored K^-1 on every pd_get_stats) */
 } pd_stats;
+
+/* Condition number from which a shape tensor counts as ill-conditioned.  The
+   mixed-precision step forms F - I = D K^-1 from FP32 sums, whose relative
+   error along K's weak direction grows as about 6e-8 * cond(K): from 1e4 on,
+   PD_MIXED can drift from PD_FP64 by more than 1e-4 and the FP64 path is the
+   one to use. */
+#define PD_ILL_CONDITIONED 1e4
 
 typedef struct pd_context pd_context;
 

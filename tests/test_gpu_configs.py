@@ -120,6 +120,7 @@ def test_full_length_runs_mixed_vs_fp64(pdpkg, name, steps):
         ctx = w.context(precision=prec)
         for _ in range(steps // 500):
             ctx.step(500)
+        assert ctx.stats().n_ill_conditioned == 0   # every K within mixed precision's range
         out.append((ctx.get_state(), ctx.get_broken()))
         ctx.close()
     (a, ba), (m, bm) = out

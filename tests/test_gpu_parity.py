@@ -442,15 +442,30 @@ def test_families_of_60_in_64_slot_tiles(pdpkg):
     w = pdpkg.plates.single_plate("h44", 22, 20, 30.0, horizon=4.4, seed=6, notch_len_cells=8)
     assert np.diff(w.row).max() == 60
     mask = w.code >= 0
-    for prec, steps, tiles, tol in ((0, 200, (6401, 16401), 1e-10),
-                                    (1, 50, (6401, 16401, 3202, 13202), 1e-5)):
+    for prec, steps, tiles, tol, n_ill in ((0, 200, (6401, 16401), 1e-10, 4),
+                                           (1, 50, (6401, 16401, 3202, 13202), 1e-5, 0)):
         u, v, f, br = run_plate_oracle(w, 0.004, steps)
         assert br.sum() > w.broken.sum() + 500   # 898 broken by the notch
+        assert ill_conditioned_oracle(w, u, br) == n_ill
         for tile in tiles:
-            st, brg, _ = run_plate_gpu(pdpkg.pdhip, w, 0.004, steps, precision=prec, tile=tile)
+            st, brg, stats = run_plate_gpu(pdpkg.pdhip, w, 0.004, steps, precision=prec, tile=tile)
             np.testing.assert_array_equal(brg, br)
             _, nonbc, _ = H.parity_norms(st["v"], v, mask)
             assert nonbc < tol, (prec, tile, nonbc)
+            assert stats.n_ill_conditioned == n_ill, (prec, tile, stats.n_ill_conditioned)
+
+
+def ill_conditioned_oracle(w, u, br, cmin=1e4):
+    """Alive bonds whose K has condition number >= cmin, from the oracle's K^-1
+    (eigenvalues by numpy)."""
+    p = O.or_params(horizon=w.horizon, dx=w.dx, rho=8000.0, dt=1e-8, E=192e9, nu=0.3, s0=0.0)
+    _, _, Ki, _ = O.forces(p, w.X, w.V, w.row, w.col, u, broken=br, rev=O.build_rev(w.row, w.col),
+                           want_Kinv=True)
+    Ki = Ki.reshape(-1, 3)
+    keep = (br == 0) & np.any(Ki != 0.0, axis=1)
+    m = np.stack([Ki[keep, 0], Ki[keep, 1], Ki[keep, 1], Ki[keep, 2]], axis=1).reshape(-1, 2, 2)
+    lam = np.abs(np.linalg.eigvalsh(m))
+    return int(np.sum(lam.max(axis=1) >= cmin * lam.min(axis=1)))
 
 
 def test_isolated_particles_and_tiny_families(pdpkg):
