@@ -644,7 +644,13 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     constexpr int NQ = MF64 ? FS / 4 : 1;
     constexpr int WF = MF64 ? FPW : 1;
     const int lk = lane & 15, lq = lane >> 4, lc = lane & 3;
-    double wc[WF][NQ][NS];   // FP64 matrix-core weight cache
+    // The FP64 matrix-core tile keeps its weights for contraction 2 (WF x NQ x
+    // NS per lane) in 32-slot families; 64-slot families evaluate them again
+    // there (measured: 2.47 -> 1.52 ms at 28-bond families, 48.6 -> 45.1 ms at
+    // 48-bond families), while the 32-slot tile loses (0.92 -> 1.16 ms) although
+    // its occupancy rises from 4 to 7 waves per SIMD.
+    constexpr bool MFRW = MF64 && FS >= 64;
+    double wc[MFRW ? 1 : WF][NQ][NS];   // FP64 matrix-core weight cache
     int Mf[WF];              // family sizes (wave-uniform)
     // Families of 64 slots evaluate the weights again in contraction 2 rather
     // than keep a row of 64 weights live per lane: the occupancy gained
@@ -700,13 +706,13 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                         if (16 * s < Mf[f]) {
                             const double om = weight<double, N2_IS_10>(hk[s], yk[s], pk[s], qk[s], ex, ey,
                                                                        pe, qe, a.n2);
-                            wc[f][q][s] = om;
+                            if constexpr (!MFRW) wc[f][q][s] = om;
                             Dacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(Pa, om, Dacc[f][s], 0, 0, 0);
                         } else {
-                            wc[f][q][s] = 0.0;
+                            if constexpr (!MFRW) wc[f][q][s] = 0.0;
                         }
                     }
-                } else {
+                } else if constexpr (!MFRW) {
 #pragma unroll
                     for (int s = 0; s < NS; ++s) wc[f][q][s] = 0.0;
                 }
@@ -852,16 +858,41 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
         double Sacc[WF][NS];
 #pragma unroll
         for (int f = 0; f < WF; ++f) {
+            // the records' geometry values (0..3) are still in place
+            double hk[NS], yk[NS], pk[NS], qk[NS];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) Sacc[f][s] = 0.0;
+            for (int s = 0; s < NS; ++s) {
+                Sacc[f][s] = 0.0;
+                if constexpr (MFRW) {
+                    const int kb = 16 * s + lk;
+                    hk[s] = 0.5 * rec[rec_at<R, FPW>(kb, f, 0)];
+                    yk[s] = 0.5 * rec[rec_at<R, FPW>(kb, f, 1)];
+                    pk[s] = rec[rec_at<R, FPW>(kb, f, 2)];
+                    qk[s] = rec[rec_at<R, FPW>(kb, f, 3)];
+                }
+            }
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 if (4 * q < Mf[f]) {
-                    const double Qa = rec[rec_at<R, FPW>(4 * q + lq, f, 4 + lc)];
+                    const int lb = 4 * q + lq;
+                    const double Qa = rec[rec_at<R, FPW>(lb, f, 4 + lc)];
+                    double ex = 0.0, ey = 0.0, pe = 0.0, qe = 0.0;
+                    if constexpr (MFRW) {
+                        ex = rec[rec_at<R, FPW>(lb, f, 0)];
+                        ey = rec[rec_at<R, FPW>(lb, f, 1)];
+                        pe = rec[rec_at<R, FPW>(lb, f, 2)];
+                        qe = rec[rec_at<R, FPW>(lb, f, 3)];
+                    }
 #pragma unroll
                     for (int s = 0; s < NS; ++s)
-                        if (16 * s < Mf[f])
-                            Sacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(Qa, wc[f][q][s], Sacc[f][s], 0, 0, 0);
+                        if (16 * s < Mf[f]) {
+                            double om;
+                            if constexpr (MFRW)
+                                om = weight<double, N2_IS_10>(hk[s], yk[s], pk[s], qk[s], ex, ey, pe, qe, a.n2);
+                            else
+                                om = wc[f][q][s];
+                            Sacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(Qa, om, Sacc[f][s], 0, 0, 0);
+                        }
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
