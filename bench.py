@@ -12,7 +12,8 @@ value  = bond-evals/s = (unbroken directed bonds) x steps / time, FP64.
 Also reported: particle-steps/s, the same plate in mixed precision (FP32 bond
 arithmetic, FP64 K and state) with its per-field error against the FP64 run,
 per-kernel times from HIP events recorded on the launch stream inside the timed
-region, the roofline of the dominant (bond) kernel, and the CPU baseline: the
+region, the roofline of the dominant (bond) kernel (HBM bytes, and beside it its
+arithmetic: the (k, l) pairs' FP64 / FP32 operations), and the CPU baseline: the
 literal single-threaded restatement of the reference's computeVelocity
 (src/main_seq.cpp:396-470) timed on a bounded sample of the same plate.
 """
@@ -51,6 +52,22 @@ def phase_bytes(n, nb, mixed):
     4, and writes a and u 16 each."""
     bond, _ = kernel_bytes(n, nb, mixed)
     return {"bond": bond, "gather": 84 * n + (8 if mixed else 16) * nb, "positions": 84 * n}
+
+
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64, vector and matrix
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (dense)
+FLOPS_PER_PAIR = 28        # per (i, k, l): weight 12, D += w P 8, S += w Q 8
+
+
+def compute_roofline(row, bond_ms, mixed):
+    """The bond kernel's arithmetic side: sum_i |H_i|^2 (k, l) pairs per launch,
+    FLOPS_PER_PAIR operations each (FMA = 2), over the kernel time, against the
+    precision's peak."""
+    pairs = float(np.sum(np.diff(row).astype(np.float64) ** 2))
+    ach = pairs * FLOPS_PER_PAIR / (bond_ms * 1e-3) / 1e12
+    peak = FP32_PEAK_TFLOPS if mixed else FP64_PEAK_TFLOPS
+    return {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+            "pairs": pairs, "flops_per_pair": FLOPS_PER_PAIR}
 
 
 def phase_fracs(pb, ms):
@@ -228,6 +245,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "kernel": "k_bond<double>",
                      "traffic": read_traffic(w.name, "fp64")},
+        "compute_roofline": compute_roofline(w.row, r["bond_ms"], False),
     }
     if "mixed" in res:
         m = res["mixed"]
@@ -250,6 +268,7 @@ def main():
             "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach_m / HBM_PEAK_GBS, "kernel": "k_bond<float>",
                          "traffic": read_traffic(w.name, "mixed")},
+            "compute_roofline": compute_roofline(w.row, m["bond_ms"], True),
             "rel_err_vs_fp64": err,
             "broken_equal_count": int(m["n_broken"]) == int(r["n_broken"]),
         }
