@@ -399,6 +399,25 @@ __device__ __forceinline__ int wave_max_family(int m)
     return __builtin_amdgcn_readfirstlane(m);
 }
 
+// An alive, non-singular K whose condition number reaches the guard's bound,
+// read from its FP64 inverse (same eigenvectors, reciprocal eigenvalues): for
+// eigenvalues l1 >= l2 > 0 and c = l1 / l2, tr^2 / det = c + 2 + 1/c.
+__device__ __forceinline__ bool kinv_guarded(const double* k, double illr)
+{
+    const double det = k[0] * k[2] - k[1] * k[1], tr = k[0] + k[2];
+    const bool zero = k[0] == 0.0 && k[1] == 0.0 && k[2] == 0.0;
+    return !zero && (det <= 0.0 || tr * tr >= illr * det);
+}
+
+// Whether any lane of the caller's family (LPF lanes, aligned) has x set.
+template <int LPF>
+__device__ __forceinline__ bool family_any(bool x, int slot)
+{
+    const unsigned long long b = __ballot(x);
+    const unsigned long long m = LPF == 64 ? ~0ull : ((1ull << LPF) - 1) << (slot * LPF);
+    return (b & m) != 0;
+}
+
 // K^-1 storage: FP64 values rounded to the bond arithmetic's precision (the
 // mixed-precision kernel rounds them to FP32 before use anyway).
 template <typename KT>
@@ -435,6 +454,7 @@ __global__ void __launch_bounds__(512) k_shape(BondArgs a)
     assemble_kinv<LPF, BPL, N2_IS_10>(smem + wib * Tile<LPF, BPL>::LDS_DOUBLES, F, S, alive, g, a.n2,
                                       a.ktol, kinv, sing);
     int nsing = 0;
+    bool guard = false;
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         if (S[s].act) {
@@ -442,10 +462,15 @@ __global__ void __launch_bounds__(512) k_shape(BondArgs a)
                 store_kinv<float>(a, S[s].b, kinv[s]);
             else
                 store_kinv<double>(a, S[s].b, kinv[s]);
+            guard = guard || kinv_guarded(kinv[s], a.illr);
         }
         nsing += __popcll(__ballot(sing[s]));
     }
     if (lane == 0 && nsing) atomicAdd(a.counters + 1, (unsigned int)nsing);
+    if (a.famflag) {
+        guard = family_any<LPF>(guard, F.slot);
+        if (F.famok && F.t == 0) a.famflag[F.fam] = guard ? 1 : 0;
+    }
 }
 
 // Global inputs of one family group for one lane, gathered in three dependent
@@ -464,15 +489,18 @@ template <typename R, int BPL> struct FamLoad {
     double2 Xj[BPL], uj[BPL];
     double Vj[BPL];
     double kap[BPL], dm[BPL];  // damage threshold and value (damage kernels)
+    unsigned char ff;          // the family's guard flag (mixed precision)
+    static constexpr bool GUARD = sizeof(R) == 4;
 };
 
 template <int LPF, typename FL>
 __device__ __forceinline__ void stage_rows(const BondArgs& a, FL& L, int lane, int group)
 {
     L.F = load_family<LPF>(a, lane, group);
+    if constexpr (FL::GUARD) L.ff = a.famflag[L.F.famok ? L.F.fam : 0];
 }
 
-template <typename R, int LPF, int BPL, int DMG = 0>
+template <typename R, int LPF, int BPL, int DMG = 0, int FIX = 0>
 __device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<R, BPL>& L)
 {
     family_extent(L.F);
@@ -487,9 +515,14 @@ __device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<R, BPL>& L
         L.jr[s] = a.col[b];
         L.br[s] = a.broken[b];
         L.rv[s] = a.rev[b];
-        L.kraw[s][0] = kv[3 * b];
-        L.kraw[s][1] = kv[3 * b + 1];
-        L.kraw[s][2] = kv[3 * b + 2];
+        if constexpr (FIX) {
+            // the FP64 fix-up assembles K itself (the stored K^-1 is FP32)
+            L.kraw[s][0] = L.kraw[s][1] = L.kraw[s][2] = R(0);
+        } else {
+            L.kraw[s][0] = kv[3 * b];
+            L.kraw[s][1] = kv[3 * b + 1];
+            L.kraw[s][2] = kv[3 * b + 2];
+        }
         if constexpr (DMG) {
             L.kap[s] = a.kappa[b];
             L.dm[s] = a.dmg[b];
@@ -553,10 +586,15 @@ template <typename R, int MF> struct Contraction {
 // re-assembly for families that lost a bond, then the two family contractions
 // and the force-state outputs.  hook_a runs halfway through contraction 1,
 // hook_b after it (a persistent wave issues its next group's loads there).
-template <typename R, int LPF, int BPL, int N2_IS_10, int MF, int DMG, typename HA, typename HB>
+// FIX = 1: the FP64 re-evaluation of a guarded family of a mixed-precision
+// context (k_bond_fix): K is assembled here (force_k), nothing is stored but the
+// T terms (as FP32, the context's W type) and f_own.
+template <typename R, int LPF, int BPL, int N2_IS_10, int MF, int DMG, typename HA, typename HB,
+          int FIX = 0>
 __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L, double* rec64,
                                           int lane, const HA& hook_a, const HB& hook_b)
 {
+    static_assert(!FIX || (sizeof(R) == 8 && !DMG), "the fix-up is FP64 without damage");
     finish_loads(L);
     constexpr bool MF32 = Contraction<R, MF>::MF32;
     constexpr bool MF64 = Contraction<R, MF>::MF64;
@@ -581,31 +619,48 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
         }
         nnew += __popcll(__ballot(newly));
     }
-    if (lane == 0 && nnew) atomicAdd(a.counters, (unsigned int)nnew);
+    if (!FIX && lane == 0 && nnew) atomicAdd(a.counters, (unsigned int)nnew);
 
     // ---- K_ik^-1: stored, or re-assembled in FP64 when the family changed ----
     double kinv[BPL][3];
+    bool guard = false;   // mixed precision: the family is evaluated again in FP64
+    if constexpr (FamLoad<R, BPL>::GUARD) guard = L.ff != 0;
     if (nnew != 0 || a.force_k) {
         Geo<double> g64[BPL];
 #pragma unroll
         for (int s = 0; s < BPL; ++s) g64[s] = geometry64(a, S[s]);
         bool sing[BPL];
         assemble_kinv<LPF, BPL, N2_IS_10>(rec64, F, S, alive, g64, a.n2, a.ktol, kinv, sing);
-        int nsing = 0;
+        if constexpr (!FIX) {
+            int nsing = 0;
+            bool g = false;
 #pragma unroll
-        for (int s = 0; s < BPL; ++s) {
-            if (S[s].act) {
-                store_kinv<R>(a, S[s].b, kinv[s]);
+            for (int s = 0; s < BPL; ++s) {
+                if (S[s].act) {
+                    store_kinv<R>(a, S[s].b, kinv[s]);
+                    g = g || kinv_guarded(kinv[s], a.illr);
+                }
+                nsing += __popcll(__ballot(sing[s]));
             }
-            nsing += __popcll(__ballot(sing[s]));
+            if (lane == 0 && nsing) atomicAdd(a.counters + 1, (unsigned int)nsing);
+            if constexpr (FamLoad<R, BPL>::GUARD) {
+                guard = family_any<LPF>(g, F.slot);
+                if (F.famok && F.t == 0 && guard != (L.ff != 0)) a.famflag[F.fam] = guard ? 1 : 0;
+            }
         }
-        if (lane == 0 && nsing) atomicAdd(a.counters + 1, (unsigned int)nsing);
     } else {
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
             kinv[s][0] = S[s].act ? double(L.kraw[s][0]) : 0.0;
             kinv[s][1] = S[s].act ? double(L.kraw[s][1]) : 0.0;
             kinv[s][2] = S[s].act ? double(L.kraw[s][2]) : 0.0;
+        }
+    }
+
+    if constexpr (FamLoad<R, BPL>::GUARD) {
+        if (a.famlist && guard && F.famok && F.t == 0) {
+            const unsigned int pos = atomicAdd(a.famcnt, 1u);
+            if (pos < (unsigned int)a.n) a.famlist[pos] = F.fam;
         }
     }
 
@@ -1012,7 +1067,10 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             R2 t;
             t.x = T0[s] * R(F.Vi);
             t.y = T1[s] * R(F.Vi);
-            reinterpret_cast<R2*>(a.W)[L.rv[s]] = t;
+            if constexpr (FIX)
+                reinterpret_cast<float2*>(a.W)[L.rv[s]] = make_float2(float(t.x), float(t.y));
+            else
+                reinterpret_cast<R2*>(a.W)[L.rv[s]] = t;
         }
     }
 }
@@ -1075,6 +1133,40 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
         cur = nxt;
         g = gn;
         wave_sync();
+    }
+}
+
+// FP64 re-evaluation of the guarded families a mixed-precision k_bond launch
+// collected (a.famlist, a.famcnt[0] entries): one family per wave on the
+// 64-lane tile, K assembled in FP64, the FP32 T terms and f_own overwritten.
+// Launched right after that k_bond on the same stream, with a.s0 = 0 (the
+// bonds broke there) and a.force_k = 1.  Every wave reads the count first; the
+// last workgroup to finish clears the list for the next launch.
+template <int N2_IS_10>
+__global__ void __launch_bounds__(256) k_bond_fix(BondArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int wpb = blockDim.x >> 6;
+    const unsigned int c0 = a.famcnt[0];
+    const unsigned int cnt = c0 < (unsigned int)a.n ? c0 : (unsigned int)a.n;
+    for (unsigned int e = blockIdx.x * wpb + wib; e < cnt; e += gridDim.x * wpb) {
+        FamLoad<double, 1> L;
+        L.F = load_family<64>(a, lane, a.famlist[e]);
+        stage_cols<double, 64, 1, 0, 1>(a, L);
+        stage_bonds<double, 64, 1>(a, L);
+        bond_body<double, 64, 1, N2_IS_10, 0, 0, NoHook, NoHook, 1>(
+            a, L, smem + wib * Tile<64, 1>::LDS_DOUBLES, lane, NoHook(), NoHook());
+        wave_sync();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(a.famcnt + 1, 1u) == gridDim.x - 1) {
+            atomicExch(a.famcnt, 0u);
+            atomicExch(a.famcnt + 1, 0u);
+        }
     }
 }
 

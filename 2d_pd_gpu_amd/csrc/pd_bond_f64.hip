@@ -1,6 +1,7 @@
 // pd_bond_f64.hip -- FP64 bond-kernel variants: tiles (32,1), (64,1), (16,2),
 // (16,3) with VALU contractions, (32,1) and (64,1) with matrix-core
-// contractions, each with and without the damage update.
+// contractions, each with and without the damage update; the FP64 fix-up of
+// the guarded families of a mixed-precision step.
 #include "pd_bond_launch.h"
 
 namespace pd {
@@ -26,6 +27,22 @@ hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg,
         if (bpl == 3 && lpf == 16) return launch_bond_n2<double, 16, 3, 0>(a, n2_is_10, wpb, pgrid, s);
     }
     return hipErrorInvalidValue;
+}
+
+// The FP64 fix-up of a mixed-precision step (k_bond_fix): a fixed grid of
+// FIX_GRID workgroups of 4 waves, so the launch does not depend on how many
+// families the preceding k_bond collected.
+constexpr int FIX_GRID = 64;
+
+hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, hipStream_t s)
+{
+    if (a.n <= 0) return hipSuccess;
+    const size_t lds = 4 * Tile<64, 1>::LDS_DOUBLES * sizeof(double);
+    if (n2_is_10)
+        hipLaunchKernelGGL(k_bond_fix<1>, dim3(FIX_GRID), dim3(256), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_bond_fix<0>, dim3(FIX_GRID), dim3(256), lds, s, a);
+    return hipGetLastError();
 }
 
 }  // namespace pd

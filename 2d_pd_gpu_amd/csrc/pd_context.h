@@ -48,6 +48,11 @@ struct pd_context {
     unsigned char* broken = nullptr;
     void* W = nullptr;
     unsigned int* counters = nullptr;
+    // mixed-precision FP64 guard (PD_OPT_MIXED_GUARD)
+    unsigned char* famflag = nullptr;   // per family, maintained by every K assembly
+    int* famlist = nullptr;             // guarded families collected by one bond launch
+    unsigned int* famcnt = nullptr;     // [0] list entries, [1] fix-up workgroups done
+    long guard_cond = 1000;             // 0: off
     int n_groups = 0;
     bool has_bc = false;
     // execution

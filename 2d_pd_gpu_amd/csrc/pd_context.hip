@@ -30,7 +30,7 @@ static void free_all(pd_context* c)
     if (c->graph) hipGraphDestroy(c->graph);
     void* ptrs[] = {c->X, c->u, c->v, c->a, c->f, c->fown, c->v0, c->V, c->E, c->pf1, c->kinv,
                     c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->W, c->counters,
-                    c->kappa, c->dmg, c->dstat};
+                    c->kappa, c->dmg, c->dstat, c->famflag, c->famlist, c->famcnt};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -75,6 +75,11 @@ static BondArgs bond_args(pd_context* c)
     a.kappa = c->kappa;
     a.dmg = c->dmg;
     a.dstat = c->dstat;
+    a.famflag = c->famflag;
+    a.famlist = nullptr;
+    a.famcnt = c->famcnt;
+    const double gc = c->guard_cond > 0 ? (double)c->guard_cond : 1e300;
+    a.illr = gc + 2.0 + 1.0 / gc;
     return a;
 }
 
@@ -248,6 +253,10 @@ static int finish_create(pd_context* c)
         (rc = dalloc(c, &c->pf1, 1)) || (rc = dalloc(c, (char**)&c->W, wbytes * (size_t)(nb > 0 ? nb : 1))) ||
         (rc = dalloc(c, &c->counters, 4)) || (rc = dalloc(c, &c->bc, n)) || (rc = dalloc(c, &c->v0, 1)))
         return rc;
+    if (c->p.precision == PD_MIXED &&
+        ((rc = dalloc(c, &c->famflag, n > 0 ? n : 1)) || (rc = dalloc(c, &c->famlist, n > 0 ? n : 1)) ||
+         (rc = dalloc(c, &c->famcnt, 2))))
+        return rc;
     hipError_t e = hipSuccess;
     if (nb > 0) {
         e = e != hipSuccess ? e : hipMemset(c->broken, 0, nb);
@@ -259,6 +268,10 @@ static int finish_create(pd_context* c)
     e = e != hipSuccess ? e : hipMemset(c->f, 0, sizeof(double2) * n);
     e = e != hipSuccess ? e : hipMemset(c->fown, 0, sizeof(double2) * n);
     e = e != hipSuccess ? e : hipMemset(c->counters, 0, 16);
+    if (c->famflag) {
+        e = e != hipSuccess ? e : hipMemset(c->famflag, 0, n > 0 ? n : 1);
+        e = e != hipSuccess ? e : hipMemset(c->famcnt, 0, 2 * sizeof(unsigned int));
+    }
     e = e != hipSuccess ? e : hipMemset(c->kinv, 0, kinv_bytes(c));
     if (nb == 0) {
         // the bond kernel reads entry 0 of the bond arrays for idle lanes
@@ -546,6 +559,19 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
         c->mf = mf;
         break;
     }
+    case PD_OPT_MIXED_GUARD: {
+        if (value < 0) return fail(c, PD_ERR_INVALID, "guard bound must be >= 0 (0: off)");
+        if (c->p.precision != PD_MIXED) {
+            if (value != 0) return fail(c, PD_ERR_INVALID, "the FP64 guard applies to PD_MIXED only");
+            break;
+        }
+        if (value != c->guard_cond) {
+            c->guard_cond = value;
+            // every family's flag under the new bound (and K^-1 as assembled)
+            if (int rc = run_shape(c)) return rc;
+        }
+        break;
+    }
     default: return fail(c, PD_ERR_INVALID, "unknown option %d", opt);
     }
     c->graph_valid = false;
@@ -678,7 +704,18 @@ static int force_and_velocity(pd_context* c, int write_a, bool with_next = false
     const PosArgs q = pos_args(c, true);
     int mixed = c->p.precision == PD_MIXED;
     const int dmg = c->dmgp.mode != PD_DAMAGE_OFF;
-    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->mf, dmg, c->n2_is_10, c->wpb, c->persist * c->n_cu, c->stream); }))
+    const bool guard = mixed && c->guard_cond > 0;
+    if (guard) a.famlist = c->famlist;
+    if (int rc = timed(c, 1, [&]() {
+            hipError_t e = launch_bond(a, mixed, c->lpf, c->bpl, c->mf, dmg, c->n2_is_10, c->wpb,
+                                       c->persist * c->n_cu, c->stream);
+            if (e != hipSuccess || !guard) return e;
+            // the guarded families again in FP64 (their bonds broke above)
+            BondArgs fx = a;
+            fx.s0 = 0.0;
+            fx.force_k = 1;
+            return launch_bond_fix(fx, c->n2_is_10, c->stream);
+        }))
         return rc;
     return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream, with_next ? &q : nullptr); });
 }
@@ -945,6 +982,13 @@ extern "C" int pd_get_stats(pd_context* c, pd_stats* s)
         HIPCHK(c, hipMemcpyAsync(&nill, c->counters + 3, sizeof(nill), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         s->n_ill_conditioned = nill;
+    }
+    if (c->famflag && c->guard_cond > 0 && c->n > 0) {
+        std::vector<unsigned char> ff(c->n);
+        HIPCHK(c, hipMemcpy(ff.data(), c->famflag, c->n, hipMemcpyDeviceToHost));
+        long ng = 0;
+        for (unsigned char x : ff) ng += x != 0;
+        s->n_guarded_families = ng;
     }
     for (int k = 0; k < 4; ++k) {
         s->phase_ms[k] = c->phase_ms[k];

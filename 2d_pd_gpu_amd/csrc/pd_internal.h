@@ -42,6 +42,13 @@ struct BondArgs {
     double* kappa;            // per bond threshold
     double* dmg;              // per bond damage
     double* dstat;            // per particle damageStatus
+    // Mixed-precision guard: a family with an alive bond whose K has a condition
+    // number of at least the guard's bound is evaluated again in FP64 after the
+    // mixed kernel (k_bond_fix), which overwrites its T terms and f_own.
+    unsigned char* famflag;   // per family: 1 = guarded (maintained by every K assembly)
+    int* famlist;             // guarded families collected this launch (nullptr: none)
+    unsigned int* famcnt;     // [0] entries in famlist, [1] fix-up workgroups finished
+    double illr;              // cond(K) >= c  <=>  tr(K)^2 >= illr det(K), illr = c + 2 + 1/c
 };
 
 struct GatherArgs {
@@ -92,6 +99,7 @@ hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double
                                    double2* a, hipStream_t s);
 hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
+hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, hipStream_t s);
 hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32, const unsigned char* broken,
                                        double cmin, unsigned int* count, hipStream_t s);
 // *first = smallest index with a non-finite u, v or f (0xffffffff if none).

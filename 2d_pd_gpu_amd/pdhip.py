@@ -37,6 +37,7 @@ OPT_XCD_REMAP = 4
 OPT_TILE = 5
 OPT_PERSISTENT = 6
 OPT_CHECK_FINITE = 7
+OPT_MIXED_GUARD = 8
 
 DAMAGE_OFF = 0
 DAMAGE_REFERENCE = 1
@@ -100,7 +101,7 @@ class Stats(C.Structure):
         ("max_family", C.c_int), ("lanes_per_family", C.c_int), ("bonds_per_lane", C.c_int),
         ("persistent", C.c_int), ("waves_per_wg", C.c_int), ("steps", C.c_long),
         ("phase_ms", C.c_double * 4), ("phase_calls", C.c_long * 4), ("matrix_cores", C.c_int),
-        ("n_ill_conditioned", C.c_long),
+        ("n_ill_conditioned", C.c_long), ("n_guarded_families", C.c_long),
     ]
 
 

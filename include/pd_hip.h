@@ -77,6 +77,12 @@ extern "C" {
 #define PD_OPT_CHECK_FINITE  7  /* 1: pd_step returns PD_ERR_NOT_FINITE, naming the
                                    first particle, when u, v or f is not finite after
                                    its steps (default 1; one reduction per call)     */
+#define PD_OPT_MIXED_GUARD   8  /* PD_MIXED only: condition-number bound c of the
+                                   FP64 guard.  A family with an alive bond whose K
+                                   has cond(K) >= c is evaluated again in FP64 after
+                                   the FP32 bond kernel, every step (FP32 sums err
+                                   by about 6e-8 cond(K) along K's weak direction).
+                                   Default 1000; 0 turns the guard off              */
 
 /* ---- Mazars bond damage (pd_set_damage) ------------------------------------ */
 #define PD_DAMAGE_OFF        0
@@ -122,6 +128,7 @@ typedef struct pd_stats {
     int    matrix_cores;   /* 1: family contractions on the matrix cores (MFMA) */
     long   n_ill_conditioned; /* alive bonds whose K has condition number >= PD_ILL_CONDITIONED
                                  now (counted from the stored K^-1 on every pd_get_stats) */
+    long   n_guarded_families; /* PD_MIXED: families the FP64 guard re-evaluates each step */
 } pd_stats;
 
 /* Condition number from which a shape tensor counts as ill-conditioned.  The

@@ -430,29 +430,88 @@ def test_delta_4015_family_of_48(pdpkg):
 
 def test_families_of_60_in_64_slot_tiles(pdpkg):
     """The largest families the tiles take: delta = 4.4 dx gives 60 bonds per
-    interior family (64-slot tiles) against the oracle, with bonds breaking.
+    interior family (64-slot tiles) against the oracle, with bonds breaking for
+    200 steps, until 7784 of the 21872 bonds are broken.
 
-    FP64 runs 200 steps, until 7784 of the 21872 bonds are broken.  Mixed
-    precision is checked after 50 steps (1594 broken, largest condition number
-    of an alive bond's K 8.4e3).  From about step 100 on, a few bonds are left
-    with nearly collinear neighbours and K condition numbers up to 4.3e9,
-    which FP32 products cannot resolve (their error grows as 6e-8 x cond), so
-    mixed precision leaves the 1e-5 bound there by design."""
+    From about step 100 on a few bonds are left with nearly collinear
+    neighbours and K condition numbers up to 4.3e9, which FP32 sums cannot
+    resolve (their error grows as 6e-8 x cond(K)).  Mixed precision keeps the
+    1e-5 bound and the FP64 broken set there through its FP64 guard (the
+    families holding such a bond are evaluated again in FP64 every step); with
+    the guard off it is checked over the first 50 steps (largest cond(K) of an
+    alive bond 8.4e3), where it holds without it."""
     say("delta = 4.4 dx, families of 60, 64-slot tiles")
+    pdh = pdpkg.pdhip
     w = pdpkg.plates.single_plate("h44", 22, 20, 30.0, horizon=4.4, seed=6, notch_len_cells=8)
     assert np.diff(w.row).max() == 60
     mask = w.code >= 0
-    for prec, steps, tiles, tol, n_ill in ((0, 200, (6401, 16401), 1e-10, 4),
-                                           (1, 50, (6401, 16401, 3202, 13202), 1e-5, 0)):
+    cases = ((0, 200, (6401, 16401), 1e-10, 4, None),
+             (1, 200, (6401, 16401, 3202, 13202), 1e-5, 4, 1000),
+             (1, 50, (6401, 16401, 3202, 13202), 1e-5, 0, 0))
+    for prec, steps, tiles, tol, n_ill, guard in cases:
         u, v, f, br = run_plate_oracle(w, 0.004, steps)
         assert br.sum() > w.broken.sum() + 500   # 898 broken by the notch
         assert ill_conditioned_oracle(w, u, br) == n_ill
         for tile in tiles:
-            st, brg, stats = run_plate_gpu(pdpkg.pdhip, w, 0.004, steps, precision=prec, tile=tile)
+            ctx = w.context(precision=prec, s0=0.004)
+            ctx.set_option(pdh.OPT_TILE, tile)
+            if guard is not None:
+                ctx.set_option(pdh.OPT_MIXED_GUARD, guard)
+            ctx.step(steps)
+            st, brg, stats = ctx.get_state(), ctx.get_broken(), ctx.stats()
+            ctx.close()
             np.testing.assert_array_equal(brg, br)
             _, nonbc, _ = H.parity_norms(st["v"], v, mask)
-            assert nonbc < tol, (prec, tile, nonbc)
+            assert nonbc < tol, (prec, steps, tile, nonbc)
             assert stats.n_ill_conditioned == n_ill, (prec, tile, stats.n_ill_conditioned)
+            if guard:
+                # the families holding the ill-conditioned bonds, and those with
+                # 1e3 <= cond(K) < 1e4
+                assert stats.n_guarded_families >= n_ill
+
+
+def test_mixed_guard_off_loses_the_fragmented_plate(pdpkg):
+    """The same plate at 200 steps with the guard off: the FP32 error exceeds
+    the bound, which is what the guard is for (a control for the test above)."""
+    say("delta = 4.4 dx, mixed precision without its FP64 guard")
+    pdh = pdpkg.pdhip
+    w = pdpkg.plates.single_plate("h44", 22, 20, 30.0, horizon=4.4, seed=6, notch_len_cells=8)
+    u, v, f, br = run_plate_oracle(w, 0.004, 200)
+    ctx = w.context(precision=1, s0=0.004)
+    ctx.set_option(pdh.OPT_TILE, 6401)
+    ctx.set_option(pdh.OPT_MIXED_GUARD, 0)
+    ctx.step(200)
+    st, stats = ctx.get_state(), ctx.stats()
+    ctx.close()
+    _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
+    assert nonbc > 1e-4
+    assert stats.n_guarded_families == 0
+
+
+def test_mixed_guard_graph_eager_chunks_bitwise(pdpkg):
+    """With guarded families, the graph-replayed step, the eager one and the
+    same steps in chunks give identical bits (the FP64 fix-up of each family is
+    deterministic and does not depend on the order the families were collected
+    in), and so does the per-phase profiled path."""
+    say("mixed guard: graph vs eager vs chunks vs profiled, bitwise")
+    pdh = pdpkg.pdhip
+    w = pdpkg.plates.single_plate("h44", 22, 20, 30.0, horizon=4.4, seed=6, notch_len_cells=8)
+    runs = []
+    for graph, chunks, profile in ((1, 1, 0), (0, 1, 0), (1, 4, 0), (1, 1, 1)):
+        ctx = w.context(precision=1, s0=0.004)
+        ctx.set_option(pdh.OPT_TILE, 3202)
+        ctx.set_option(pdh.OPT_USE_GRAPH, graph)
+        ctx.set_option(pdh.OPT_PROFILE, profile)
+        for _ in range(chunks):
+            ctx.step(200 // chunks)
+        runs.append((ctx.get_state(), ctx.get_broken(), ctx.stats().n_guarded_families))
+        ctx.close()
+    assert runs[0][2] > 0
+    for st, b, ng in runs[1:]:
+        for k in ("u", "v", "a", "f"):
+            np.testing.assert_array_equal(st[k], runs[0][0][k])
+        np.testing.assert_array_equal(b, runs[0][1])
+        assert ng == runs[0][2]
 
 
 def ill_conditioned_oracle(w, u, br, cmin=1e4):
