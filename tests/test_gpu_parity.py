@@ -514,6 +514,34 @@ def test_mixed_guard_graph_eager_chunks_bitwise(pdpkg):
         assert ng == runs[0][2]
 
 
+def test_fracturing_plate_mixed_tracks_fp64_until_it_shatters(pdpkg):
+    """A 192 x 192 plate (centre notch, +-20 m/s, s0 = 0.006, 5 % E scatter)
+    whose load waves meet at the centre line after about 2000 steps and shatter
+    it.  Up to step 1500 the guarded mixed run keeps the FP64 velocities within
+    1e-5 and the FP64 broken set; after that the fracture amplifies any
+    perturbation (two FP64 tiles part there too), so the guarded run is only
+    required to stay finite through step 3000 (every pd_step checks)."""
+    say("fracturing 192 x 192 plate: guarded mixed vs FP64")
+    pdh = pdpkg.pdhip
+    w = pdpkg.plates.single_plate("frac192", 192, 192, 20.0, notch_len_cells=48, seed=11, E_sigma=0.05)
+    out = []
+    for prec in (0, 1):
+        ctx = w.context(precision=prec, s0=0.006)
+        ctx.set_option(pdh.OPT_TILE, 1602)
+        ctx.step(1500)
+        out.append((ctx, ctx.get_state(), ctx.get_broken()))
+    (c64, a, ba), (cm, m, bm) = out
+    assert ba.sum() > w.broken.sum() + 1000
+    np.testing.assert_array_equal(bm, ba)
+    _, nonbc, _ = H.parity_norms(m["v"], a["v"], w.code >= 0)
+    assert nonbc < 1e-5, nonbc
+    c64.close()
+    cm.step(1500)                      # through the shattering: stays finite
+    st = cm.stats()
+    assert st.n_broken > 10 * ba.sum() and st.n_guarded_families > 0
+    cm.close()
+
+
 def ill_conditioned_oracle(w, u, br, cmin=1e4):
     """Alive bonds whose K has condition number >= cmin, from the oracle's K^-1
     (eigenvalues by numpy)."""
