@@ -1150,6 +1150,9 @@ __global__ void __launch_bounds__(256) k_bond_fix(BondArgs a)
     const int wib = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
     const unsigned int c0 = a.famcnt[0];
+    // an empty list (the usual case) leaves nothing to clear: every workgroup
+    // reads the same count, so all of them return here together
+    if (c0 == 0) return;
     const unsigned int cnt = c0 < (unsigned int)a.n ? c0 : (unsigned int)a.n;
     for (unsigned int e = blockIdx.x * wpb + wib; e < cnt; e += gridDim.x * wpb) {
         FamLoad<double, 1> L;
