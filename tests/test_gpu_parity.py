@@ -542,6 +542,61 @@ def test_fracturing_plate_mixed_tracks_fp64_until_it_shatters(pdpkg):
     cm.close()
 
 
+def test_guard_delta_4015_heavy_fracture_against_oracle(pdpkg):
+    """delta = 4.015 dx (families of 48) with s0 = 0.004 for 300 steps, 13046
+    bonds broken: the guarded mixed tiles, including the three-bonds-per-lane
+    ones, keep the oracle's broken set and 1e-5; without the guard the same run
+    leaves both."""
+    say("delta = 4.015 dx heavy fracture: guarded mixed tiles vs the oracle")
+    pdh = pdpkg.pdhip
+    w = pdpkg.plates.single_plate("d4", 48, 40, 30.0, horizon=4.015, seed=3, notch_len_cells=16)
+    u, v, f, br = run_plate_oracle(w, 0.004, 300)
+    assert br.sum() > 10000
+    mask = w.code >= 0
+    for tile in (1603, 11603, 3202, 6401):
+        st, brg, stats = run_plate_gpu(pdh, w, 0.004, 300, precision=1, tile=tile)
+        np.testing.assert_array_equal(brg, br)
+        _, nonbc, _ = H.parity_norms(st["v"], v, mask)
+        assert nonbc < 1e-5, (tile, nonbc)
+        assert stats.n_guarded_families > 0
+    ctx = w.context(precision=1, s0=0.004)
+    ctx.set_option(pdh.OPT_TILE, 1603)
+    ctx.set_option(pdh.OPT_MIXED_GUARD, 0)
+    ctx.step(300)
+    st = ctx.get_state()
+    ctx.close()
+    _, nonbc, _ = H.parity_norms(st["v"], v, mask)
+    assert nonbc > 1e-4
+
+
+def test_guard_persistent_kernels_bitwise(pdpkg):
+    """The persistent bond kernels collect the guarded families from their
+    prefetched groups: on the fracturing 192 x 192 plate (over 200 guarded
+    families at step 2600) they give the plain launch's bits."""
+    say("guarded families: persistent vs plain bond kernel, bitwise")
+    pdh = pdpkg.pdhip
+    w = pdpkg.plates.single_plate("frac192", 192, 192, 20.0, notch_len_cells=48, seed=11, E_sigma=0.05)
+
+    def run(tile, persist, wpb):
+        ctx = w.context(precision=1, s0=0.006)
+        ctx.set_option(pdh.OPT_TILE, tile)
+        ctx.set_option(pdh.OPT_WAVES_PER_WG, wpb)
+        ctx.set_option(pdh.OPT_PERSISTENT, persist)
+        ctx.step(2600)
+        r = ctx.get_state(), ctx.get_broken(), ctx.stats().n_guarded_families
+        ctx.close()
+        return r
+
+    for tile in (1602, 1603):
+        a = run(tile, 0, 4)
+        assert a[2] > 100
+        for persist, wpb in ((1, 4), (8, 1)):
+            b = run(tile, persist, wpb)
+            for k in ("u", "v", "a", "f"):
+                np.testing.assert_array_equal(b[0][k], a[0][k])
+            np.testing.assert_array_equal(b[1], a[1])
+
+
 def ill_conditioned_oracle(w, u, br, cmin=1e4):
     """Alive bonds whose K has condition number >= cmin, from the oracle's K^-1
     (eigenvalues by numpy)."""
