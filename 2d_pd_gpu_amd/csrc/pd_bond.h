@@ -1137,11 +1137,10 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
 }
 
 // FP64 re-evaluation of the guarded families a mixed-precision k_bond launch
-// collected (a.famlist, a.famcnt[0] entries): one family per wave on the
-// 64-lane tile, K assembled in FP64, the FP32 T terms and f_own overwritten.
-// Launched right after that k_bond on the same stream, with a.s0 = 0 (the
-// bonds broke there) and a.force_k = 1.  Every wave reads the count first; the
-// last workgroup to finish clears the list for the next launch.
+// collected (a.famlist, *a.famcnt entries): one family per wave on the 64-lane
+// tile, K assembled in FP64, the FP32 T terms and f_own overwritten.  Launched
+// right after that k_bond on the same stream, with a.s0 = 0 (the bonds broke
+// there) and a.force_k = 1; the gather that follows clears the count.
 template <int N2_IS_10>
 __global__ void __launch_bounds__(256) k_bond_fix(BondArgs a)
 {
@@ -1149,10 +1148,7 @@ __global__ void __launch_bounds__(256) k_bond_fix(BondArgs a)
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
     const int wpb = blockDim.x >> 6;
-    const unsigned int c0 = a.famcnt[0];
-    // an empty list (the usual case) leaves nothing to clear: every workgroup
-    // reads the same count, so all of them return here together
-    if (c0 == 0) return;
+    const unsigned int c0 = *a.famcnt;
     const unsigned int cnt = c0 < (unsigned int)a.n ? c0 : (unsigned int)a.n;
     for (unsigned int e = blockIdx.x * wpb + wib; e < cnt; e += gridDim.x * wpb) {
         FamLoad<double, 1> L;
@@ -1162,14 +1158,6 @@ __global__ void __launch_bounds__(256) k_bond_fix(BondArgs a)
         bond_body<double, 64, 1, N2_IS_10, 0, 0, NoHook, NoHook, 1>(
             a, L, smem + wib * Tile<64, 1>::LDS_DOUBLES, lane, NoHook(), NoHook());
         wave_sync();
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(a.famcnt + 1, 1u) == gridDim.x - 1) {
-            atomicExch(a.famcnt, 0u);
-            atomicExch(a.famcnt + 1, 0u);
-        }
     }
 }
 

@@ -29,19 +29,17 @@ hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg,
     return hipErrorInvalidValue;
 }
 
-// The FP64 fix-up of a mixed-precision step (k_bond_fix): a fixed grid of
-// FIX_GRID workgroups of 4 waves, so the launch does not depend on how many
-// families the preceding k_bond collected.
-constexpr int FIX_GRID = 64;
-
-hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, hipStream_t s)
+// The FP64 fix-up of a mixed-precision step (k_bond_fix): a grid of `grid`
+// workgroups of 4 waves, sized to the device by the caller, so the launch does
+// not depend on how many families the preceding k_bond collected.
+hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, int grid, hipStream_t s)
 {
-    if (a.n <= 0) return hipSuccess;
+    if (a.n <= 0 || grid <= 0) return hipSuccess;
     const size_t lds = 4 * Tile<64, 1>::LDS_DOUBLES * sizeof(double);
     if (n2_is_10)
-        hipLaunchKernelGGL(k_bond_fix<1>, dim3(FIX_GRID), dim3(256), lds, s, a);
+        hipLaunchKernelGGL(k_bond_fix<1>, dim3(grid), dim3(256), lds, s, a);
     else
-        hipLaunchKernelGGL(k_bond_fix<0>, dim3(FIX_GRID), dim3(256), lds, s, a);
+        hipLaunchKernelGGL(k_bond_fix<0>, dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

@@ -51,7 +51,7 @@ struct pd_context {
     // mixed-precision FP64 guard (PD_OPT_MIXED_GUARD)
     unsigned char* famflag = nullptr;   // per family, maintained by every K assembly
     int* famlist = nullptr;             // guarded families collected by one bond launch
-    unsigned int* famcnt = nullptr;     // [0] list entries, [1] fix-up workgroups done
+    unsigned int* famcnt = nullptr;     // list entries (cleared by the gather)
     long guard_cond = 1000;             // 0: off
     int n_groups = 0;
     bool has_bc = false;

@@ -97,6 +97,7 @@ static GatherArgs gather_args(pd_context* c, int write_a)
     g.rho = c->p.rho;
     g.dt = c->p.dt;
     g.write_a = write_a;
+    g.famcnt = nullptr;
     return g;
 }
 
@@ -255,7 +256,7 @@ static int finish_create(pd_context* c)
         return rc;
     if (c->p.precision == PD_MIXED &&
         ((rc = dalloc(c, &c->famflag, n > 0 ? n : 1)) || (rc = dalloc(c, &c->famlist, n > 0 ? n : 1)) ||
-         (rc = dalloc(c, &c->famcnt, 2))))
+         (rc = dalloc(c, &c->famcnt, 1))))
         return rc;
     hipError_t e = hipSuccess;
     if (nb > 0) {
@@ -270,7 +271,7 @@ static int finish_create(pd_context* c)
     e = e != hipSuccess ? e : hipMemset(c->counters, 0, 16);
     if (c->famflag) {
         e = e != hipSuccess ? e : hipMemset(c->famflag, 0, n > 0 ? n : 1);
-        e = e != hipSuccess ? e : hipMemset(c->famcnt, 0, 2 * sizeof(unsigned int));
+        e = e != hipSuccess ? e : hipMemset(c->famcnt, 0, sizeof(unsigned int));
     }
     e = e != hipSuccess ? e : hipMemset(c->kinv, 0, kinv_bytes(c));
     if (nb == 0) {
@@ -705,7 +706,10 @@ static int force_and_velocity(pd_context* c, int write_a, bool with_next = false
     int mixed = c->p.precision == PD_MIXED;
     const int dmg = c->dmgp.mode != PD_DAMAGE_OFF;
     const bool guard = mixed && c->guard_cond > 0;
-    if (guard) a.famlist = c->famlist;
+    if (guard) {
+        a.famlist = c->famlist;
+        g.famcnt = c->famcnt;
+    }
     if (int rc = timed(c, 1, [&]() {
             hipError_t e = launch_bond(a, mixed, c->lpf, c->bpl, c->mf, dmg, c->n2_is_10, c->wpb,
                                        c->persist * c->n_cu, c->stream);
@@ -714,7 +718,7 @@ static int force_and_velocity(pd_context* c, int write_a, bool with_next = false
             BondArgs fx = a;
             fx.s0 = 0.0;
             fx.force_k = 1;
-            return launch_bond_fix(fx, c->n2_is_10, c->stream);
+            return launch_bond_fix(fx, c->n2_is_10, 4 * c->n_cu, c->stream);
         }))
         return rc;
     return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream, with_next ? &q : nullptr); });

@@ -47,7 +47,7 @@ struct BondArgs {
     // mixed kernel (k_bond_fix), which overwrites its T terms and f_own.
     unsigned char* famflag;   // per family: 1 = guarded (maintained by every K assembly)
     int* famlist;             // guarded families collected this launch (nullptr: none)
-    unsigned int* famcnt;     // [0] entries in famlist, [1] fix-up workgroups finished
+    unsigned int* famcnt;     // entries in famlist (cleared by the gather that follows)
     double illr;              // cond(K) >= c  <=>  tr(K)^2 >= illr det(K), illr = c + 2 + 1/c
 };
 
@@ -62,6 +62,8 @@ struct GatherArgs {
     double2* f;
     double rho, dt;
     int write_a;
+    unsigned int* famcnt;     // mixed-precision guard: the list count the preceding
+                              // k_bond / k_bond_fix used, cleared here (nullptr: none)
 };
 
 struct PosArgs {
@@ -99,7 +101,7 @@ hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double
                                    double2* a, hipStream_t s);
 hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
-hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, hipStream_t s);
+hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, int grid, hipStream_t s);
 hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32, const unsigned char* broken,
                                        double cmin, unsigned int* count, hipStream_t s);
 // *first = smallest index with a non-finite u, v or f (0xffffffff if none).

@@ -87,6 +87,8 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
         f0 += __shfl_xor(f0, off, 64);
         f1 += __shfl_xor(f1, off, 64);
     }
+    // the guard's list is used up (k_bond_fix ran before this kernel)
+    if (g.famcnt && gid == 0) *g.famcnt = 0u;
     if (!ok || t != 0) return;
     {
 #pragma clang fp contract(off)
