@@ -28,17 +28,26 @@ __device__ __forceinline__ void position_update(const PosArgs& p, int i, double2
 }
 
 // f_i = sum_j T_ij V_j - sum_j T_ji V_i ; v += 1/2 (a + f/rho) dt.
-// GATHER_LANES lanes per particle read its W row with unit stride (coalesced),
+// GatherShape<R>::LANES lanes per particle read its W row with unit stride (coalesced),
 // then reduce across the group.  POS: the step is followed by another one in
 // the same pd_step call, so the next step's position update (k_positions)
 // runs here on the final f and v, with the same operations in the same order.
-constexpr int GATHER_LANES = 8;
-constexpr int GATHER_UNROLL = 4;
+// Lanes per particle and the number of row entries each lane loads at once:
+// 8 x 4 for FP64 rows (16-byte entries), 4 x 7 for FP32 rows (8-byte entries;
+// 28 entries, an interior family at delta = 3.015, in one round).  Measured on
+// the 1024 x 1024 plate, unfused gather: FP32 4 x 7 64-65 us, 4 x 8 65-66 us,
+// 8 x 4 70-71 us, 2 x 16 83 us; FP64 8 x 4 101 us, 4 x 8 108 us.
+template <typename R> struct GatherShape {
+    static constexpr int LANES = sizeof(R) == 8 ? 8 : 4;
+    static constexpr int UNROLL = sizeof(R) == 8 ? 4 : 7;
+};
 
 template <typename R, bool POS>
 __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
 {
     typedef typename Vec2<R>::T R2;
+    constexpr int GATHER_LANES = GatherShape<R>::LANES;
+    constexpr int GATHER_UNROLL = GatherShape<R>::UNROLL;
     const int blk = xcd_remap(blockIdx.x, gridDim.x, g.xcd);
     const int gid = blk * blockDim.x + threadIdx.x;
     const int i = gid / GATHER_LANES;
@@ -237,7 +246,7 @@ hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int w
 
 hipError_t launch_gather(const GatherArgs& g, int mixed, hipStream_t s, const PosArgs* next)
 {
-    const int grid = ceil_div((long)g.n * GATHER_LANES, 256);
+    const int grid = ceil_div((long)g.n * (mixed ? GatherShape<float>::LANES : GatherShape<double>::LANES), 256);
     if (grid == 0) return hipSuccess;
     const PosArgs p = next ? *next : PosArgs{};
     if (mixed) {
