@@ -107,8 +107,10 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
     double2 v;
     v.x = vi.x + (0.5 * (a0.x + an0)) * g.dt;
     v.y = vi.y + (0.5 * (a0.y + an1)) * g.dt;
-    g.f[i] = make_double2(f0, f1);
     if constexpr (POS) {
+        // f is not stored here: nothing reads it before the last step of the
+        // pd_step call, whose gather (POS = false) stores it; the next step
+        // needs only a = f / rho.
         // the next step's position update (k_positions), same operations
         if (code >= 0) v = p.v0[code];
         g.v[i] = v;
@@ -120,6 +122,7 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
         if (code == -2) u = make_double2(0.0, 0.0);
         p.u[i] = u;
     } else {
+        g.f[i] = make_double2(f0, f1);
         g.v[i] = v;
         if (g.write_a) g.a[i] = make_double2(an0, an1);
     }
