@@ -97,6 +97,7 @@ static GatherArgs gather_args(pd_context* c, int write_a)
     g.rho = c->p.rho;
     g.dt = c->p.dt;
     g.write_a = write_a;
+    g.max_family = c->maxM;
     g.famcnt = nullptr;
     return g;
 }

@@ -62,6 +62,7 @@ struct GatherArgs {
     double2* f;
     double rho, dt;
     int write_a;
+    int max_family;           // largest family: picks the launch shape
     unsigned int* famcnt;     // mixed-precision guard: the list count the preceding
                               // k_bond / k_bond_fix used, cleared here (nullptr: none)
 };

@@ -28,26 +28,22 @@ __device__ __forceinline__ void position_update(const PosArgs& p, int i, double2
 }
 
 // f_i = sum_j T_ij V_j - sum_j T_ji V_i ; v += 1/2 (a + f/rho) dt.
-// GatherShape<R>::LANES lanes per particle read its W row with unit stride (coalesced),
+// GATHER_LANES lanes per particle read its W row with unit stride (coalesced),
 // then reduce across the group.  POS: the step is followed by another one in
 // the same pd_step call, so the next step's position update (k_positions)
 // runs here on the final f and v, with the same operations in the same order.
-// Lanes per particle and the number of row entries each lane loads at once:
-// 8 x 4 for FP64 rows (16-byte entries), 4 x 7 for FP32 rows (8-byte entries;
-// 28 entries, an interior family at delta = 3.015, in one round).  Measured on
-// the 1024 x 1024 plate, unfused gather: FP32 4 x 7 64-65 us, 4 x 8 65-66 us,
-// 8 x 4 70-71 us, 2 x 16 83 us; FP64 8 x 4 101 us, 4 x 8 108 us.
-template <typename R> struct GatherShape {
-    static constexpr int LANES = sizeof(R) == 8 ? 8 : 4;
-    static constexpr int UNROLL = sizeof(R) == 8 ? 4 : 7;
-};
-
-template <typename R, bool POS>
+// LANES lanes per particle read its W row; each lane first loads UNROLL entries
+// of its share at once, then loops over the rest.  The launch picks the shape
+// from the largest family: FP32 rows of up to 28 entries (an interior family at
+// delta = 3.015) take 4 x 7, one round; otherwise 8 lanes with 4, 6 or 8 loads
+// in flight, so the row fits one round.  Measured, unfused gather: 1024 x 1024
+// plate (families of 28) FP32 4 x 7 64-65 us, 4 x 8 65-66 us, 8 x 4 70-71 us,
+// 2 x 16 83 us, FP64 8 x 4 101 us, 4 x 8 108 us; 4096 x 4096 plate (families
+// of 48) FP32 8 x 4 1.56 ms, 4 x 7 1.89 ms.
+template <typename R, bool POS, int GATHER_LANES, int GATHER_UNROLL>
 __global__ void __launch_bounds__(256) k_gather(GatherArgs g, PosArgs p)
 {
     typedef typename Vec2<R>::T R2;
-    constexpr int GATHER_LANES = GatherShape<R>::LANES;
-    constexpr int GATHER_UNROLL = GatherShape<R>::UNROLL;
     const int blk = xcd_remap(blockIdx.x, gridDim.x, g.xcd);
     const int gid = blk * blockDim.x + threadIdx.x;
     const int i = gid / GATHER_LANES;
@@ -247,23 +243,31 @@ hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int w
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_gather(const GatherArgs& g, int mixed, hipStream_t s, const PosArgs* next)
+template <typename R, int LANES, int UNROLL>
+static hipError_t launch_gather_t(const GatherArgs& g, hipStream_t s, const PosArgs* next)
 {
-    const int grid = ceil_div((long)g.n * (mixed ? GatherShape<float>::LANES : GatherShape<double>::LANES), 256);
+    const int grid = ceil_div((long)g.n * LANES, 256);
     if (grid == 0) return hipSuccess;
     const PosArgs p = next ? *next : PosArgs{};
-    if (mixed) {
-        if (next)
-            hipLaunchKernelGGL((k_gather<float, true>), dim3(grid), dim3(256), 0, s, g, p);
-        else
-            hipLaunchKernelGGL((k_gather<float, false>), dim3(grid), dim3(256), 0, s, g, p);
-    } else {
-        if (next)
-            hipLaunchKernelGGL((k_gather<double, true>), dim3(grid), dim3(256), 0, s, g, p);
-        else
-            hipLaunchKernelGGL((k_gather<double, false>), dim3(grid), dim3(256), 0, s, g, p);
-    }
+    if (next)
+        hipLaunchKernelGGL((k_gather<R, true, LANES, UNROLL>), dim3(grid), dim3(256), 0, s, g, p);
+    else
+        hipLaunchKernelGGL((k_gather<R, false, LANES, UNROLL>), dim3(grid), dim3(256), 0, s, g, p);
     return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_gather_r(const GatherArgs& g, hipStream_t s, const PosArgs* next)
+{
+    if (sizeof(R) == 4 && g.max_family <= 28) return launch_gather_t<R, 4, 7>(g, s, next);
+    if (g.max_family <= 32) return launch_gather_t<R, 8, 4>(g, s, next);
+    if (g.max_family <= 48) return launch_gather_t<R, 8, 6>(g, s, next);
+    return launch_gather_t<R, 8, 8>(g, s, next);
+}
+
+hipError_t launch_gather(const GatherArgs& g, int mixed, hipStream_t s, const PosArgs* next)
+{
+    return mixed ? launch_gather_r<float>(g, s, next) : launch_gather_r<double>(g, s, next);
 }
 
 hipError_t launch_positions(const PosArgs& p, hipStream_t s)
