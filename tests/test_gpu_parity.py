@@ -486,6 +486,17 @@ def test_mixed_guard_off_loses_the_fragmented_plate(pdpkg):
     _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
     assert nonbc > 1e-4
     assert stats.n_guarded_families == 0
+    # the option's error paths: FP64 contexts have no guard, bounds are >= 0
+    ctx = w.context(precision=0, s0=0.004)
+    ctx.set_option(pdh.OPT_MIXED_GUARD, 0)
+    with pytest.raises(pdh.PDError):
+        ctx.set_option(pdh.OPT_MIXED_GUARD, 1000)
+    assert ctx.stats().n_guarded_families == 0
+    ctx.close()
+    ctx = w.context(precision=1, s0=0.004)
+    with pytest.raises(pdh.PDError):
+        ctx.set_option(pdh.OPT_MIXED_GUARD, -1)
+    ctx.close()
 
 
 def test_mixed_guard_graph_eager_chunks_bitwise(pdpkg):
