@@ -95,6 +95,33 @@ def test_gathered_file_with_damage(pdpkg, tmp_path):
     assert status.max() > 0
 
 
+def test_checkpoint_resume_with_guarded_families_is_bitwise(pdpkg, tmp_path):
+    """Mixed precision on the fragmenting delta = 4.4 dx plate: checkpoint at
+    step 100 (families already guarded), 100 more steps, against a fresh
+    context that loads it (its guard flags rebuilt from the broken set) and
+    runs the same 100: identical state and broken set."""
+    say("checkpoint / resume with guarded families")
+    w = pdpkg.plates.single_plate("h44", 22, 20, 30.0, horizon=4.4, seed=6, notch_len_cells=8)
+    ck = tmp_path / "guard.ckpt"
+    ctx = w.context(precision=1, s0=0.004)
+    ctx.step(100)
+    assert ctx.stats().n_guarded_families > 0
+    ctx.save_checkpoint(str(ck))
+    ctx.step(100)
+    a_state, a_br, a_ng = ctx.get_state(), ctx.get_broken(), ctx.stats().n_guarded_families
+    ctx.close()
+    ctx = w.context(precision=1, s0=0.004)
+    ctx.load_checkpoint(str(ck))
+    assert ctx.stats().n_guarded_families > 0
+    ctx.step(100)
+    b_state, b_br, b_ng = ctx.get_state(), ctx.get_broken(), ctx.stats().n_guarded_families
+    ctx.close()
+    for k in ("u", "v", "a", "f"):
+        np.testing.assert_array_equal(a_state[k], b_state[k])
+    np.testing.assert_array_equal(a_br, b_br)
+    assert a_ng == b_ng
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_checkpoint_resume_is_bitwise(pdpkg, tmp_path, precision):
     """30 steps, checkpoint, 30 more, against a fresh context that loads the
