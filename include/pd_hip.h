@@ -133,9 +133,10 @@ typedef struct pd_stats {
 
 /* Condition number from which a shape tensor counts as ill-conditioned.  The
    mixed-precision step forms F - I = D K^-1 from FP32 sums, whose relative
-   error along K's weak direction grows as about 6e-8 * cond(K): from 1e4 on,
-   PD_MIXED can drift from PD_FP64 by more than 1e-4 and the FP64 path is the
-   one to use. */
+   error along K's weak direction grows as about 6e-8 * cond(K).  The FP64
+   guard (PD_OPT_MIXED_GUARD, bound 1000 by default) re-evaluates the families
+   holding such bonds in FP64; with the guard off, PD_MIXED can drift from
+   PD_FP64 by more than 1e-4 once bonds reach this bound. */
 #define PD_ILL_CONDITIONED 1e4
 
 typedef struct pd_context pd_context;
