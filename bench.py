@@ -208,7 +208,8 @@ def main():
         kb, sb = kernel_bytes(w.n, w.nb, prec == pkg.MIXED)
         res[name] = dict(t=t_graph, t_prof=t_prof, nb_alive=nb_alive, bond_ms=bond_ms,
                          gather_ms=gather_ms, pos_ms=pos_ms, kb=kb, sb=sb, state=state,
-                         n_broken=st.n_broken, tile=(st.lanes_per_family, st.bonds_per_lane, st.matrix_cores,
+                         n_broken=st.n_broken, guarded=st.n_guarded_families,
+                         tile=(st.lanes_per_family, st.bonds_per_lane, st.matrix_cores,
                                st.waves_per_wg, st.persistent))
         print(f"# {name}: {args.steps} steps graph {t_graph:.3f}s eager+events {t_prof:.3f}s "
               f"bond {bond_ms:.4f} ms gather {gather_ms:.4f} ms positions {pos_ms:.4f} ms "
@@ -271,6 +272,7 @@ def main():
             "compute_roofline": compute_roofline(w.row, m["bond_ms"], True),
             "rel_err_vs_fp64": err,
             "broken_equal_count": int(m["n_broken"]) == int(r["n_broken"]),
+            "guarded_families": int(m["guarded"]),
         }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pkg, w, r["state"])
