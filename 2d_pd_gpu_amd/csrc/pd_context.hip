@@ -590,6 +590,10 @@ extern "C" int pd_set_youngs_modulus(pd_context* c, const double* E)
         if (!(e > 0)) return fail(c, PD_ERR_INVALID, "E[%d] must be positive", c->perm[q]);
         Ei[q] = e / (1.0 - c->p.nu * c->p.nu);
     }
+    hipSetDevice(c->dev);
+    // the copy below uses the null stream, which does not wait for the context's
+    // non-blocking stream
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!c->E)
         if (int rc = dalloc(c, &c->E, c->n)) return rc;
     HIPCHK(c, hipMemcpy(c->E, Ei.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
@@ -734,6 +738,9 @@ extern "C" int pd_set_damage(pd_context* c, const pd_damage_params* d)
     if (d->mode != PD_DAMAGE_OFF && d->mode != PD_DAMAGE_REFERENCE && d->mode != PD_DAMAGE_MAZARS)
         return fail(c, PD_ERR_INVALID, "unknown damage mode %d", d->mode);
     hipSetDevice(c->dev);
+    // copies below use the null stream, which does not wait for the context's
+    // non-blocking stream
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (d->mode != PD_DAMAGE_OFF) {
         if (!(d->kappa0 > 0) || !std::isfinite(d->At) || !std::isfinite(d->Bt) || !std::isfinite(d->Ac) ||
             !std::isfinite(d->Bc))
@@ -826,6 +833,9 @@ extern "C" int pd_set_boundary(pd_context* c, const int* code, int n_groups, con
 {
     if (!c || !code || n_groups < 0 || (n_groups > 0 && !v0)) return fail(c, PD_ERR_INVALID, "bad boundary program");
     hipSetDevice(c->dev);
+    // copies below use the null stream, which does not wait for the context's
+    // non-blocking stream
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<int> ci(c->n);
     for (int q = 0; q < c->n; ++q) {
         int g = code[c->perm[q]];
@@ -923,6 +933,9 @@ extern "C" int pd_set_broken(pd_context* c, const unsigned char* mask)
 {
     if (!c || !mask) return fail(c, PD_ERR_INVALID, "NULL argument");
     hipSetDevice(c->dev);
+    // copies below use the null stream, which does not wait for the context's
+    // non-blocking stream
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<unsigned char> t(c->nb);
     for (long b = 0; b < c->nb; ++b) t[b] = mask[c->bmap[b]] ? 1 : 0;
     for (long b = 0; b < c->nb; ++b)
@@ -936,6 +949,9 @@ extern "C" int pd_get_bond_forces(pd_context* c, double* T)
 {
     if (!c || !T) return fail(c, PD_ERR_INVALID, "NULL argument");
     hipSetDevice(c->dev);
+    // copies below use the null stream, which does not wait for the context's
+    // non-blocking stream
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<double> w(2 * (size_t)c->nb);
     if (c->nb) {
         if (c->p.precision == PD_MIXED) {
@@ -963,6 +979,9 @@ extern "C" int pd_get_stats(pd_context* c, pd_stats* s)
 {
     if (!c || !s) return fail(c, PD_ERR_INVALID, "NULL argument");
     hipSetDevice(c->dev);
+    // copies below use the null stream, which does not wait for the context's
+    // non-blocking stream
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (int rc = drain_events(c)) return rc;
     memset(s, 0, sizeof(*s));
     s->n = c->n;
