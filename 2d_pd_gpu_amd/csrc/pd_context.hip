@@ -1042,16 +1042,17 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     const size_t n = c->n, nb = c->nb;
     const bool dmg = c->kappa != nullptr;
     double2 *su = nullptr, *sv = nullptr, *sa = nullptr, *sf = nullptr;
-    unsigned char* sb = nullptr;
+    unsigned char *sb = nullptr, *sff = nullptr;
     double *sk = nullptr, *skap = nullptr, *sdm = nullptr, *sst = nullptr;
     auto cleanup = [&]() {
         for (void* q : {(void*)su, (void*)sv, (void*)sa, (void*)sf, (void*)sb, (void*)sk, (void*)skap,
-                        (void*)sdm, (void*)sst})
+                        (void*)sdm, (void*)sst, (void*)sff})
             if (q) hipFree(q);
     };
     int rc;
     if ((rc = dalloc(c, &su, n)) || (rc = dalloc(c, &sv, n)) || (rc = dalloc(c, &sa, n)) ||
         (rc = dalloc(c, &sf, n)) || (rc = dalloc(c, &sb, nb)) || (rc = dalloc(c, (char**)&sk, kinv_bytes(c))) ||
+        (c->famflag && (rc = dalloc(c, &sff, n))) ||
         (dmg && ((rc = dalloc(c, &skap, nb)) || (rc = dalloc(c, &sdm, nb)) || (rc = dalloc(c, &sst, n))))) {
         cleanup();
         return rc;
@@ -1071,6 +1072,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
         cp(c->f, sf, 16 * n);
         cp(c->broken, sb, nb);
         cp(c->kinv, sk, kinv_bytes(c));
+        if (c->famflag) cp(c->famflag, sff, n);   // the guard flags follow K
         if (dmg) {
             cp(c->kappa, skap, 8 * nb);
             cp(c->dmg, sdm, 8 * nb);

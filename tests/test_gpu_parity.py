@@ -608,6 +608,25 @@ def test_guard_persistent_kernels_bitwise(pdpkg):
             np.testing.assert_array_equal(b[1], a[1])
 
 
+def test_autotune_restores_guard_flags(pdpkg):
+    """pd_autotune times every variant on the context's own state, whose trial
+    steps break bonds and re-assemble K (and with it the guard flags): the state,
+    the broken set and the guarded families are as before the call."""
+    say("autotune restores state and guard flags")
+    w = pdpkg.plates.single_plate("h44", 22, 20, 30.0, horizon=4.4, seed=6, notch_len_cells=8)
+    ctx = w.context(precision=1, s0=0.004)
+    ctx.step(150)
+    st0, br0, ng0 = ctx.get_state(), ctx.get_broken(), ctx.stats().n_guarded_families
+    assert ng0 > 0
+    ctx.autotune(5)
+    st1, br1, ng1 = ctx.get_state(), ctx.get_broken(), ctx.stats().n_guarded_families
+    ctx.close()
+    for k in ("u", "v", "a", "f"):
+        np.testing.assert_array_equal(st1[k], st0[k])
+    np.testing.assert_array_equal(br1, br0)
+    assert ng1 == ng0
+
+
 def ill_conditioned_oracle(w, u, br, cmin=1e4):
     """Alive bonds whose K has condition number >= cmin, from the oracle's K^-1
     (eigenvalues by numpy)."""
