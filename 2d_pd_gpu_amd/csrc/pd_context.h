@@ -49,6 +49,7 @@ struct pd_context {
     void* W = nullptr;
     unsigned int* counters = nullptr;
     double2* stage = nullptr;           // pinned host staging for one particle vector
+    bool stage_failed = false;          // pinned allocation refused: pageable staging
     // mixed-precision FP64 guard (PD_OPT_MIXED_GUARD)
     unsigned char* famflag = nullptr;   // per family, maintained by every K assembly
     int* famlist = nullptr;             // guarded families collected by one bond launch

@@ -609,21 +609,26 @@ extern "C" int pd_set_youngs_modulus(pd_context* c, const double* E)
 // Particle vectors cross the ABI in the caller's order: the reorder runs on the
 // host threads, and the copy goes through a pinned staging buffer (allocated at
 // the first transfer, n double2), about 3x faster than from pageable memory.
-static int ensure_stage(pd_context* c)
+// If pinned memory cannot be had, a pageable buffer of the call serves instead.
+static double2* stage_buffer(pd_context* c, std::vector<double2>& fallback)
 {
-    if (c->stage || c->n == 0) return PD_OK;
-    if (hipHostMalloc((void**)&c->stage, sizeof(double2) * (size_t)c->n, hipHostMallocDefault) != hipSuccess) {
-        c->stage = nullptr;
-        return fail(c, PD_ERR_NOMEM, "pinned staging buffer of %zu bytes", sizeof(double2) * (size_t)c->n);
+    if (!c->stage && !c->stage_failed) {
+        if (hipHostMalloc((void**)&c->stage, sizeof(double2) * (size_t)c->n, hipHostMallocDefault) != hipSuccess) {
+            c->stage = nullptr;
+            c->stage_failed = true;
+            (void)hipGetLastError();   // the failed allocation is not a sticky error
+        }
     }
-    return PD_OK;
+    if (c->stage) return c->stage;
+    fallback.resize(c->n);
+    return fallback.data();
 }
 
 static int put_vec(pd_context* c, double2* dst, const double* src)
 {
     if (!src || c->n == 0) return PD_OK;
-    if (int rc = ensure_stage(c)) return rc;
-    double2* t = c->stage;
+    std::vector<double2> fb;
+    double2* t = stage_buffer(c, fb);
     const int* perm = c->perm.data();
     parallel_for(c->n, [&](long q0, long q1) {
         for (long q = q0; q < q1; ++q) {
@@ -639,8 +644,8 @@ static int put_vec(pd_context* c, double2* dst, const double* src)
 static int get_vec(pd_context* c, const double2* src, double* dst)
 {
     if (!dst || c->n == 0) return PD_OK;
-    if (int rc = ensure_stage(c)) return rc;
-    double2* t = c->stage;
+    std::vector<double2> fb;
+    double2* t = stage_buffer(c, fb);
     HIPCHK(c, hipMemcpyAsync(t, src, sizeof(double2) * c->n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const int* perm = c->perm.data();
