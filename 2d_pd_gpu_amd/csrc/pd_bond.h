@@ -588,18 +588,19 @@ template <typename R, int MF> struct Contraction {
 // hook_b after it (a persistent wave issues its next group's loads there).
 // FIX = 1: the FP64 re-evaluation of a guarded family of a mixed-precision
 // context (k_bond_fix): K is assembled here (force_k), nothing is stored but the
-// T terms (as FP32, the context's W type) and f_own.
+// T terms (as FP32, the context's W type), f_own and, with DMG, the family's
+// damage update (which the mixed kernel leaves to it).
 template <typename R, int LPF, int BPL, int N2_IS_10, int MF, int DMG, typename HA, typename HB,
           int FIX = 0>
 __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L, double* rec64,
                                           int lane, const HA& hook_a, const HB& hook_b)
 {
-    static_assert(!FIX || (sizeof(R) == 8 && !DMG), "the fix-up is FP64 without damage");
+    static_assert(!FIX || sizeof(R) == 8, "the fix-up is FP64");
     finish_loads(L);
     constexpr bool MF32 = Contraction<R, MF>::MF32;
     constexpr bool MF64 = Contraction<R, MF>::MF64;
-    static_assert(!MF64 || (BPL == 1 && (LPF == 32 || LPF == 64)),
-                  "FP64 matrix-core contraction: tiles (32,1) and (64,1)");
+    static_assert(!MF64 || (Tile<LPF, BPL>::FS % 16 == 0),
+                  "FP64 matrix-core contraction: families of a multiple of 16 slots");
     typedef typename Vec2<R>::T R2;
     constexpr int FS = Tile<LPF, BPL>::FS;
     constexpr int FPW = Tile<LPF, BPL>::FPW;
@@ -700,11 +701,12 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     constexpr int WF = MF64 ? FPW : 1;
     const int lk = lane & 15, lq = lane >> 4, lc = lane & 3;
     // The FP64 matrix-core tile keeps its weights for contraction 2 (WF x NQ x
-    // NS per lane) in 32-slot families; 64-slot families evaluate them again
-    // there (measured: 2.47 -> 1.52 ms at 28-bond families, 48.6 -> 45.1 ms at
-    // 48-bond families), while the 32-slot tile loses (0.92 -> 1.16 ms) although
-    // its occupancy rises from 4 to 7 waves per SIMD.
-    constexpr bool MFRW = MF64 && FS >= 64;
+    // NS per lane) when they are at most 32 doubles (the (32,1) tile); larger
+    // caches evaluate them again there (measured: 64-slot families 2.47 ->
+    // 1.52 ms at 28-bond families, 48.6 -> 45.1 ms at 48-bond families), while
+    // the (32,1) tile loses without its cache (0.92 -> 1.16 ms) although its
+    // occupancy rises from 4 to 7 waves per SIMD.
+    constexpr bool MFRW = MF64 && WF * NQ * NS > 32;
     double wc[MFRW ? 1 : WF][NQ][NS];   // FP64 matrix-core weight cache
     int Mf[WF];              // family sizes (wave-uniform)
     // Families of 64 slots evaluate the weights again in contraction 2 rather
@@ -860,6 +862,12 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     const R c11 = R(pf), c12 = R(pf * a.nu), c33 = R(pf * (1.0 - a.nu));
     R Q[BPL][4];
     double dsum = 0.0;
+    // A guarded family's damage is updated by the FP64 fix-up that follows,
+    // from its FP64 strain; the mixed kernel leaves its state alone (an update
+    // from the FP32 strain first would raise kappa to a wrong value that the
+    // fix-up could not lower).
+    bool dmg_here = true;
+    if constexpr (FamLoad<R, BPL>::GUARD) dmg_here = !(a.famlist && guard);
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         const R K00 = R(kinv[s][0]), K01 = R(kinv[s][1]), K11 = R(kinv[s][2]);
@@ -872,7 +880,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             // a broken bond counts as fully damaged; a singular K (K^-1 = 0)
             // gives zero strain, so its bond is left as it was
             double d = L.dm[s];
-            if (S[s].act) {
+            if (S[s].act && dmg_here) {
                 if (!alive[s]) {
                     if (d != 1.0) {
                         d = 1.0;
@@ -1059,7 +1067,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     if constexpr (DMG) {
         // damageStatus_i = sum_k d_ik / |H_i| (src/main_seq.cpp:593)
         dsum = family_sum<LPF>(dsum);
-        if (F.famok && F.t == 0) a.dstat[F.fam] = F.M > 0 ? dsum / F.M : 0.0;
+        if (F.famok && F.t == 0 && dmg_here) a.dstat[F.fam] = F.M > 0 ? dsum / F.M : 0.0;
     }
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
@@ -1138,10 +1146,11 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
 
 // FP64 re-evaluation of the guarded families a mixed-precision k_bond launch
 // collected (a.famlist, *a.famcnt entries): one family per wave on the 64-lane
-// tile, K assembled in FP64, the FP32 T terms and f_own overwritten.  Launched
-// right after that k_bond on the same stream, with a.s0 = 0 (the bonds broke
-// there) and a.force_k = 1; the gather that follows clears the count.
-template <int N2_IS_10>
+// tile, K assembled in FP64, the FP32 T terms and f_own overwritten, and with
+// DMG the family's damage updated from its FP64 strain.  Launched right after
+// that k_bond on the same stream, with a.s0 = 0 (the bonds broke there) and
+// a.force_k = 1; the gather that follows clears the count.
+template <int N2_IS_10, int DMG>
 __global__ void __launch_bounds__(256) k_bond_fix(BondArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) double smem[];
@@ -1153,9 +1162,9 @@ __global__ void __launch_bounds__(256) k_bond_fix(BondArgs a)
     for (unsigned int e = blockIdx.x * wpb + wib; e < cnt; e += gridDim.x * wpb) {
         FamLoad<double, 1> L;
         L.F = load_family<64>(a, lane, a.famlist[e]);
-        stage_cols<double, 64, 1, 0, 1>(a, L);
+        stage_cols<double, 64, 1, DMG, 1>(a, L);
         stage_bonds<double, 64, 1>(a, L);
-        bond_body<double, 64, 1, N2_IS_10, 0, 0, NoHook, NoHook, 1>(
+        bond_body<double, 64, 1, N2_IS_10, 0, DMG, NoHook, NoHook, 1>(
             a, L, smem + wib * Tile<64, 1>::LDS_DOUBLES, lane, NoHook(), NoHook());
         wave_sync();
     }

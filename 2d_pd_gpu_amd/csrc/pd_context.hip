@@ -393,6 +393,23 @@ extern "C" int pd_create(const pd_params* p, int n, const double* X, const doubl
         delete c;
         return rc;
     }
+    // The reverse-bond map is an involution exactly when no family lists a
+    // neighbour twice: a repeated (q -> r) makes both copies find the same
+    // first (r -> q), so one W slot would be written twice and another never.
+    std::atomic<long> dup{-1};
+    parallel_for(nb, [&](long b0, long b1) {
+        for (long b = b0; b < b1; ++b)
+            if (rev[rev[b]] != b) dup = b;
+    });
+    if (dup >= 0) {
+        const long b = dup;
+        const long e = c->bmap[b];
+        const int i = (int)(std::upper_bound(row_ptr, row_ptr + n + 1, (int)e) - row_ptr) - 1;
+        int rc = fail(nullptr, PD_ERR_INVALID,
+                      "family %d lists neighbour %d more than once (caller bond %ld)", i, col[e], e);
+        delete c;
+        return rc;
+    }
     c->row_host = row;
     c->rev_host = rev;
     c->X_caller.assign(X, X + 2 * (size_t)n);
@@ -757,7 +774,7 @@ static int force_and_velocity(pd_context* c, int write_a, bool with_next = false
             BondArgs fx = a;
             fx.s0 = 0.0;
             fx.force_k = 1;
-            return launch_bond_fix(fx, c->n2_is_10, 4 * c->n_cu, c->stream);
+            return launch_bond_fix(fx, c->n2_is_10, dmg, 4 * c->n_cu, c->stream);
         }))
         return rc;
     return timed(c, 2, [&]() { return launch_gather(g, mixed, c->stream, with_next ? &q : nullptr); });
@@ -806,7 +823,18 @@ extern "C" int pd_compute_damage(pd_context* c)
     BondArgs a = bond_args(c);
     a.s0 = 0.0;  // no bond breaking in a damage-only evaluation
     const int mixed = c->p.precision == PD_MIXED;
-    if (int rc = timed(c, 1, [&]() { return launch_bond(a, mixed, c->lpf, c->bpl, c->mf, 1, c->n2_is_10, c->wpb, 0, c->stream); }))
+    // mixed precision: the guarded families' damage comes from the FP64 fix-up,
+    // whose list is cleared afterwards (no gather follows here to clear it)
+    const bool guard = mixed && c->guard_cond > 0;
+    if (guard) a.famlist = c->famlist;
+    if (int rc = timed(c, 1, [&]() {
+            hipError_t e = launch_bond(a, mixed, c->lpf, c->bpl, c->mf, 1, c->n2_is_10, c->wpb, 0, c->stream);
+            if (e != hipSuccess || !guard) return e;
+            BondArgs fx = a;
+            fx.force_k = 1;
+            e = launch_bond_fix(fx, c->n2_is_10, 1, 4 * c->n_cu, c->stream);
+            return e != hipSuccess ? e : launch_fill_u32(c->famcnt, 0u, 1, c->stream);
+        }))
         return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PD_OK;

@@ -81,8 +81,8 @@ struct PosArgs {
 // Tiles: (lanes per family, bonds per lane) in {(32,1), (64,1), (16,2),
 // (16,3)} and, for mixed precision, (32,2); a family may hold up to
 // lpf * bpl bonds.
-// mf = 1: family contractions on the matrix cores (every mixed tile; FP64
-// tiles (32,1) and (64,1)).
+// mf = 1: family contractions on the matrix cores (every tile except FP64
+// (32,2), which does not exist).
 bool tile_supported(int mixed, int lpf, int bpl, int mf);
 // pgrid > 0: persistent launch with pgrid workgroups (each walks a contiguous
 // range of family groups, prefetching the next group's loads).
@@ -102,7 +102,7 @@ hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double
                                    double2* a, hipStream_t s);
 hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
-hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, int grid, hipStream_t s);
+hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, int dmg, int grid, hipStream_t s);
 hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32, const unsigned char* broken,
                                        double cmin, unsigned int* count, hipStream_t s);
 // *first = smallest index with a non-finite u, v or f (0xffffffff if none).

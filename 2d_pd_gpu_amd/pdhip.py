@@ -176,8 +176,18 @@ def _f64(a, n=None):
     return a
 
 
-def _i32(a):
-    return np.ascontiguousarray(a, dtype=np.int32)
+def _i32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _u8(a, n):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
 
 
 def _ptr(a):
@@ -229,12 +239,14 @@ class Context:
     def __init__(self, params, X, V, row, col):
         self._h = C.c_void_p()
         self.params = params
-        X = _f64(X)
         V = _f64(V)
-        row = _i32(row)
-        col = _i32(col)
         self.n = V.size
+        X = _f64(X, 2 * self.n)
+        row = _i32(row, self.n + 1)
+        if row[0] != 0 or np.any(np.diff(row) < 0):
+            raise ValueError("row_ptr must start at 0 and never decrease")
         self.nb = int(row[-1])
+        col = _i32(col, self.nb)
         rc = lib().pd_create(C.byref(params), self.n, _ptr(X), _ptr(V), _ptr(row), _ptr(col),
                              C.byref(self._h))
         if rc != PD_OK:
@@ -247,9 +259,9 @@ class Context:
         self = cls.__new__(cls)
         self._h = C.c_void_p()
         self.params = params
-        X = _f64(X)
         V = _f64(V)
         self.n = V.size
+        X = _f64(X, 2 * self.n)
         rc = lib().pd_create_from_points(C.byref(params), self.n, _ptr(X), _ptr(V), C.byref(self._h))
         if rc != PD_OK:
             raise PDError(rc, lib().pd_last_error(None).decode())
@@ -270,11 +282,11 @@ class Context:
         self = cls.__new__(cls)
         self._h = C.c_void_p()
         self.params = params
-        X = _f64(X)
         V = _f64(V)
-        nbr = _i32(nbr)
-        cnt = _i32(nbr_count)
         self.n = V.size
+        X = _f64(X, 2 * self.n)
+        cnt = _i32(nbr_count, self.n)
+        nbr = _i32(nbr, self.n * int(cap))
         self.nb = int(cnt.sum())
         rc = lib().pd_create_padded(C.byref(params), self.n, _ptr(X), _ptr(V), _ptr(nbr),
                                     _ptr(cnt), int(cap), C.byref(self._h))
@@ -349,8 +361,10 @@ class Context:
 
     # -- whole steps --
     def set_boundary(self, code, v0_groups):
-        code = _i32(code)
+        code = _i32(code, self.n)
         v0 = _f64(v0_groups)
+        if v0.size % 2:
+            raise ValueError("v0_groups holds (vx, vy) per group: an even count")
         self._check(lib().pd_set_boundary(self._h, _ptr(code), v0.size // 2, _ptr(v0)))
 
     def step(self, nsteps=1):
@@ -366,7 +380,7 @@ class Context:
         return m[: self.nb]
 
     def set_broken(self, mask):
-        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        m = _u8(mask, self.nb)
         self._check(lib().pd_set_broken(self._h, _ptr(m)))
 
     def get_bond_forces(self):
@@ -403,11 +417,11 @@ class Context:
         self._check(lib().pd_write_velocity(self._h, os.fsencode(path), int(digits)))
 
     def write_positions(self, path, ids=None, digits=0):
-        ids = None if ids is None else _i32(ids)
+        ids = None if ids is None else _i32(ids, self.n)
         self._check(lib().pd_write_positions(self._h, os.fsencode(path), _ptr(ids), int(digits)))
 
     def write_gathered(self, path, ids=None, digits=0):
-        ids = None if ids is None else _i32(ids)
+        ids = None if ids is None else _i32(ids, self.n)
         self._check(lib().pd_write_gathered(self._h, os.fsencode(path), _ptr(ids), int(digits)))
 
     # -- checkpoint / resume --

@@ -68,7 +68,7 @@ extern "C" {
 #define PD_OPT_TILE          5  /* family tile: 10000*matrix_cores + 100*lanes_per_family
                                    + bonds_per_lane: 3201, 6401, 1602, 1603 (and 3202
                                    in mixed precision); matrix-core contractions
-                                   13201, 16401 (and 11602, 13202, 11603 in mixed
+                                   13201, 16401, 11602, 11603 (and 13202 in mixed
                                    precision);
                                    0 = 3201 or 6401 by the largest family            */
 #define PD_OPT_PERSISTENT    6  /* bond kernel as a persistent grid of value workgroups

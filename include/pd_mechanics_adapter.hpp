@@ -29,6 +29,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -60,14 +61,32 @@ inline void check(int rc, pd_context* ctx)
     if (rc != PD_OK) throw std::runtime_error(std::string("lib2dpdgpu: ") + pd_last_error(ctx));
 }
 
+// FNV-1a over everything the context is built from: each particle's family
+// (neighbour ids as mapped to local indices, -1 where the map has none) and the
+// exact bits of its initial position (x and y, as the double the context
+// stores) and volume.  A mesh that differs in any of them gets a new context.
+inline uint64_t fnv_mix(uint64_t h, uint64_t x) { return (h ^ x) * 1099511628211ULL; }
+
+inline uint64_t double_bits(double d)
+{
+    uint64_t b;
+    std::memcpy(&b, &d, sizeof b);
+    return b;
+}
+
 template <class ParticleT>
-uint64_t mesh_signature(const std::vector<ParticleT>& ps)
+uint64_t mesh_signature(const std::vector<ParticleT>& ps, const std::unordered_map<int, int>& globalLocalIDmap)
 {
     uint64_t h = 1469598103934665603ULL;
     for (const auto& p : ps) {
-        h = (h ^ (uint64_t)p.neighbors.size()) * 1099511628211ULL;
-        for (int nb : p.neighbors) h = (h ^ (uint64_t)(uint32_t)nb) * 1099511628211ULL;
-        h = (h ^ (uint64_t)std::llround((double)p.initialPositions[0] * 1e9)) * 1099511628211ULL;
+        h = fnv_mix(h, (uint64_t)p.neighbors.size());
+        for (int nb : p.neighbors) {
+            auto it = globalLocalIDmap.find(nb);
+            h = fnv_mix(h, (uint64_t)(uint32_t)(it == globalLocalIDmap.end() ? -1 : it->second));
+        }
+        h = fnv_mix(h, double_bits((double)p.initialPositions[0]));
+        h = fnv_mix(h, double_bits((double)p.initialPositions[1]));
+        h = fnv_mix(h, double_bits((double)p.volume));
     }
     return h;
 }
@@ -82,7 +101,7 @@ pd_context* mesh_context(const pd_params& p, const std::vector<ParticleT>& allPa
 {
     const int n = (int)allParticles.size();
     Cache& c = cache();
-    const uint64_t sig = mesh_signature(allParticles);
+    const uint64_t sig = mesh_signature(allParticles, globalLocalIDmap);
     const bool same_mesh = c.ctx && c.n == n && c.sig == sig && c.p.n1 == p.n1 && c.p.n2 == p.n2 &&
                            c.p.horizon == p.horizon && c.p.dx == p.dx;
     const bool same = same_mesh && (geometry_only || (c.p.E == p.E && c.p.nu == p.nu &&

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "build", "liboracle.so")
+# the default build; PD_ORACLE_LIB names another build of it (the sanitized one)
+ORACLE_SO = os.environ.get("PD_ORACLE_LIB") or os.path.join(ROOT, "oracle", "build", "liboracle.so")
 REF_MATRIX_SO = os.path.join(ROOT, "oracle", "_ref", "libref_matrix.so")
 
 

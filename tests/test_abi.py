@@ -89,6 +89,73 @@ def test_create_rejects_bad_input_without_device(pdpkg):
         pdh.Context(p, X, V, row, col)
 
 
+def test_create_rejects_duplicate_neighbours_without_device(pdpkg):
+    """A family that lists a neighbour twice would map both copies to one
+    reverse slot (one force term written twice, another never): rejected,
+    naming the family, the neighbour and the caller bond, whether the
+    duplicate is one-sided or mirrored in the neighbour's family."""
+    pdh = pdpkg.pdhip
+    X = _mesh()
+    n = X.size // 2
+    V = np.ones(n)
+    row, col = pdh.build_families(X, 1.0, 3.015)
+    p = pdh.Params.make(dx=1.0)
+    i = 7
+    j = int(col[row[i]])
+    fams = [list(col[row[k]:row[k + 1]]) for k in range(n)]
+    for mirrored in (False, True):
+        f2 = [list(f) for f in fams]
+        f2[i].append(j)
+        if mirrored:
+            f2[j].append(i)
+        row2 = np.concatenate([[0], np.cumsum([len(f) for f in f2])]).astype(np.int32)
+        col2 = np.concatenate(f2).astype(np.int32)
+        with pytest.raises(pdh.PDError) as e:
+            pdh.Context(p, X, V, row2, col2)
+        assert e.value.code == pdh.PD_ERR_INVALID
+        msg = str(e.value)
+        assert "more than once" in msg and f"neighbour {j}" in msg, msg
+        assert f"family {i}" in msg or f"family {j}" in msg, msg
+
+
+def test_binding_checks_array_sizes(pdpkg):
+    """The Python binding refuses arrays of the wrong length before the C
+    library could read past their end."""
+    import ctypes
+
+    pdh = pdpkg.pdhip
+    X = _mesh()
+    n = X.size // 2
+    V = np.ones(n)
+    row, col = pdh.build_families(X, 1.0, 3.015)
+    p = pdh.Params.make(dx=1.0)
+    for args in ((X[:-2], V, row, col), (X, V, row[:-1], col), (X, V, row, col[:-1])):
+        with pytest.raises(ValueError):
+            pdh.Context(p, *args)
+    with pytest.raises(ValueError):
+        pdh.Context.from_points(p, X[:-1], V)
+    with pytest.raises(ValueError):
+        pdh.Context.from_padded(p, X, V, np.zeros(n * 30 - 1, np.int32), np.zeros(n, np.int32), 30)
+    ctx = pdh.Context.__new__(pdh.Context)   # no device context: the checks come first
+    ctx._h = ctypes.c_void_p()
+    ctx.n, ctx.nb = n, len(col)
+    with pytest.raises(ValueError):
+        ctx.set_broken(np.zeros(len(col) - 1, np.uint8))
+    with pytest.raises(ValueError):
+        ctx.set_boundary(np.full(n - 1, -1, np.int32), np.zeros(2))
+    with pytest.raises(ValueError):
+        ctx.set_boundary(np.full(n, -1, np.int32), np.zeros(3))
+    for fn in (ctx.set_youngs_modulus,):
+        with pytest.raises(ValueError):
+            fn(np.ones(n + 1))
+    with pytest.raises(ValueError):
+        ctx.set_state(u=np.zeros(2 * n - 1))
+    with pytest.raises(ValueError):
+        ctx.set_damage_state(bond_damage=np.zeros(len(col) + 1))
+    with pytest.raises(ValueError):
+        ctx.write_positions("/dev/null", ids=np.arange(n - 1))
+
+
 def test_build_families_matches_reference_find_neighbor(pdpkg):
     X, V, row_ref, col_ref, sets = H.case()
     row, col = pdpkg.build_families(X.astype(np.float64), 1.0, 3.0)
@@ -154,3 +221,39 @@ def test_read_mesh_matches_reference_reader(pdpkg, tmp_path):
     bad = tmp_path / "missing.dat"
     with pytest.raises(pdh.PDError):
         pdh.read_mesh(str(bad))
+
+
+def test_adapter_mesh_signature_sees_y_and_volume(tmp_path):
+    """The adapter reuses its device context only for an identical mesh: the
+    signature changes with a particle's y position or volume, not only with its
+    families and x."""
+    src = tmp_path / "sig.cpp"
+    src.write_text(r'''
+#include "pd_mechanics_adapter.hpp"
+#include <cstdio>
+#include <set>
+struct Particle { int globalID; long double volume; std::vector<long double> initialPositions;
+                  std::vector<int> neighbors; };
+int main() {
+    std::vector<Particle> ps(3);
+    std::unordered_map<int, int> ids;
+    for (int i = 0; i < 3; ++i) {
+        ps[i].globalID = 10 + i; ps[i].volume = 1.0L; ps[i].initialPositions = {1.0L * i, 0.0L};
+        ids[10 + i] = i;
+    }
+    ps[0].neighbors = {11}; ps[1].neighbors = {10, 12}; ps[2].neighbors = {11};
+    std::set<uint64_t> seen;
+    seen.insert(pd_adapter::mesh_signature(ps, ids));
+    auto q = ps; q[1].initialPositions[1] = 1e-12L; seen.insert(pd_adapter::mesh_signature(q, ids));
+    q = ps; q[2].volume = 1.0L + 1e-15L; seen.insert(pd_adapter::mesh_signature(q, ids));
+    q = ps; q[0].initialPositions[0] = 1e-300L; seen.insert(pd_adapter::mesh_signature(q, ids));
+    auto m = ids; m[12] = 0; seen.insert(pd_adapter::mesh_signature(ps, m));
+    bool same = pd_adapter::mesh_signature(ps, ids) == pd_adapter::mesh_signature(ps, ids);
+    std::printf("%zu %d\n", seen.size(), (int)same);
+    return (seen.size() == 5 && same) ? 0 : 1;
+}
+''')
+    exe = tmp_path / "sig"
+    rc = os.system(f"g++ -std=c++17 -I{ROOT}/include -o {exe} {src}")
+    assert rc == 0
+    assert os.system(str(exe)) == 0

@@ -46,3 +46,24 @@ def test_pairs_of_the_100_plate(pdpkg):
     row, col = pdpkg.pdhip.build_families(X, 1e-3, 3.015)
     assert len(col) == 272836
     assert bench.compute_roofline(row, 1.0, False)["pairs"] == 7503364
+
+
+def test_cpu_window_sample_is_a_symmetric_sub_mesh(pdpkg):
+    """The strong-CPU baseline's sample: the bonds between the particles of a
+    central window, so families are symmetric and those of particles further
+    than a horizon from the window's edge are the full plate's families."""
+    import oracle_lib as O
+
+    w = pdpkg.plates.plate_100()
+    Xw, Vw, roww, colw, ids = bench.window(w, 40)
+    assert len(Vw) == 1600 and np.all(np.diff(ids) > 0)
+    O.build_rev(roww, colw)  # raises on an asymmetric family
+    X = w.X.reshape(-1, 2)
+    c = np.median(X, axis=0)
+    deep = np.nonzero(np.all(np.abs(X[ids] - c) < (20 - 4) * w.dx, axis=1))[0]
+    assert len(deep) > 500
+    for q in deep:
+        i = ids[q]
+        np.testing.assert_array_equal(ids[colw[roww[q]:roww[q + 1]]], w.col[w.row[i]:w.row[i + 1]])
+    model, ncpu, usable = bench.host_cpu()
+    assert model and ncpu >= 1 and 1 <= usable <= ncpu

@@ -110,7 +110,8 @@ def test_damage_during_steps_matches_oracle(pdpkg, plate, mode):
         assert agree.mean() > 0.99
 
 
-@pytest.mark.parametrize("precision,tile", [(0, 13201), (0, 16401), (1, 11602), (1, 13201)])
+@pytest.mark.parametrize("precision,tile", [(0, 13201), (0, 16401), (0, 11603), (1, 11602),
+                                            (1, 13201)])
 def test_damage_with_matrix_core_contractions(pdpkg, plate, precision, tile):
     """The fused damage update on the matrix-core tiles, 300 steps against the
     oracle (mode 2, the published law)."""
@@ -220,3 +221,44 @@ def test_autotune_restores_damage_state(pdpkg, plate):
         np.testing.assert_array_equal(before[key], after[key])
     for key in ("u", "v", "a", "f"):
         np.testing.assert_array_equal(st0[key], st1[key])
+
+
+def test_mixed_guarded_families_damage_matches_oracle(pdpkg):
+    """Damage on a fracturing plate in mixed precision with guarded families
+    (delta = 4.015 dx, families of 48, s0 = 0.004, 300 steps, over 10,000 bonds
+    broken): the families with an ill-conditioned K take their forces and their
+    damage update from the FP64 fix-up, so per-bond kappa and d and the
+    particles' damageStatus stay within the mixed-precision bar of the FP64
+    oracle, with the oracle's broken set."""
+    say("mixed precision, guarded families, damage mode 2: delta = 4.015 plate, 300 steps")
+    pdh = pdpkg.pdhip
+    w = pdpkg.plates.single_plate("d4", 48, 40, 30.0, horizon=4.015, seed=3, notch_len_cells=16)
+    p = oracle_params(w)
+    p.s0 = 0.004
+    rev = O.build_rev(w.row, w.col)
+    u, v, a, f = (np.zeros(2 * w.n) for _ in range(4))
+    br = w.broken.copy()
+    kap = np.full(w.nb, KAPPA0)
+    dm = np.zeros(w.nb)
+    st = np.zeros(w.n)
+    dp = O.damage_params(mode=2, kappa0=KAPPA0)
+    O.step_damage(p, dp, w.X, w.V, w.row, w.col, rev, w.code, w.v0, u, v, a, f, br, kap, dm, st, 300)
+    assert br.sum() > 10000
+    for tile in (1603, 11603):
+        ctx = w.context(precision=1, s0=0.004)
+        ctx.set_option(pdh.OPT_TILE, tile)
+        ctx.set_damage(pdh.DamageParams.make(mode=2, kappa0=KAPPA0))
+        ctx.step(300)
+        g = ctx.get_damage()
+        stats = ctx.stats()
+        np.testing.assert_array_equal(ctx.get_broken(), br)
+        ctx.close()
+        assert stats.n_guarded_families > 0
+        ek = np.linalg.norm(g["kappa"] - kap) / np.linalg.norm(kap)
+        ed = np.linalg.norm(g["bond"] - dm) / np.linalg.norm(dm)
+        es = np.linalg.norm(g["status"] - st) / np.linalg.norm(st)
+        worst = np.max(np.abs(g["kappa"] - kap) / kap)
+        say(f"  tile {tile}: guarded families {stats.n_guarded_families}, kappa {ek:.2e}, "
+            f"d {ed:.2e}, status {es:.2e}, worst bond kappa {worst:.2e}")
+        assert ek < 1e-5 and ed < 1e-5 and es < 1e-5
+        assert worst < 1e-3

@@ -274,7 +274,7 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     pdh = pdpkg.pdhip
     ref, bref, _ = run_plate_gpu(pdh, small_plate, 0.005, 40)
     mask = small_plate.code >= 0
-    for tile in (3201, 6401, 1602, 1603, 13201, 16401):
+    for tile in (3201, 6401, 1602, 1603, 13201, 16401, 11602, 11603):
         tile_ref = None
         for wpb, xcd, persist in ((1, 0, 0), (2, 1, 0), (8, 0, 0), (8, 1, 0), (4, 1, 1), (2, 0, 3),
                                   (8, 1, 2)):
@@ -316,7 +316,7 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     with pytest.raises(pdh.PDError):
         ctx = small_plate.context(s0=0.005)
         try:
-            ctx.set_option(pdh.OPT_TILE, 11602)  # nor a matrix-core (16, 2) tile
+            ctx.set_option(pdh.OPT_TILE, 13202)  # nor a matrix-core (32, 2) tile
         finally:
             ctx.close()
     ctx = small_plate.context(s0=0.005)
@@ -335,7 +335,7 @@ def test_hassa_matrix_core_contractions(pdpkg, hassa_case, hassa_oracle):
     v_mfma_f32_4x4x1 in mixed precision) meet the same bars as the VALU path."""
     say("hassa single step, matrix-core contractions")
     u, v, a, f, T = hassa_oracle
-    for tile in (13201, 16401):
+    for tile in (13201, 16401, 11602, 11603):
         st = hassa_gpu(pdpkg, hassa_case, tile=tile)
         np.testing.assert_array_equal(st["u"], u)
         _, nonbc, elem = H.parity_norms(st["v"], v, bc_mask(hassa_case))
@@ -347,7 +347,7 @@ def test_hassa_matrix_core_contractions(pdpkg, hassa_case, hassa_oracle):
         assert nonbc < 1e-5
 
 
-@pytest.mark.parametrize("tile", [13201, 16401])
+@pytest.mark.parametrize("tile", [13201, 16401, 11602])
 def test_plate_matrix_core_breaking_matches_oracle(pdpkg, small_plate, small_plate_oracle, tile):
     say(f"plate 48x40, 300 steps, s0 = 0.005, matrix-core tile {tile}")
     pdh = pdpkg.pdhip
@@ -365,7 +365,8 @@ def test_plate_matrix_core_breaking_matches_oracle(pdpkg, small_plate, small_pla
     assert nonbc < 1e-5
 
 
-@pytest.mark.parametrize("precision,tile", [(0, 0), (1, 0), (0, 13201), (1, 13201), (0, 1603), (1, 11603)])
+@pytest.mark.parametrize("precision,tile", [(0, 0), (1, 0), (0, 13201), (1, 13201), (0, 1603), (1, 11603),
+                                            (0, 11603)])
 def test_persistent_bond_kernel_bitwise(pdpkg, precision, tile):
     """The persistent bond kernel (each wave walks many family groups and
     prefetches the next one) gives the same bits as one wave per group."""
@@ -407,7 +408,7 @@ def test_delta_4015_family_of_48(pdpkg):
     assert stats.lanes_per_family * stats.bonds_per_lane >= 48
     _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
     assert nonbc < 1e-11
-    for tile in (16401, 1603):
+    for tile in (16401, 1603, 11603):
         st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, tile=tile)
         _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
         assert nonbc < 1e-11
@@ -667,7 +668,7 @@ def test_isolated_particles_and_tiny_families(pdpkg):
     fx = f.reshape(-1, 2)[base.n:]
     assert np.all(fx[:5] == 0.0) and np.abs(fx[5:]).max() > 0
     mask = w.code >= 0
-    for prec, tiles, tol in ((0, (3201, 6401, 1602, 1603, 13201, 16401), 1e-11),
+    for prec, tiles, tol in ((0, (3201, 6401, 1602, 1603, 13201, 16401, 11602, 11603), 1e-11),
                              (1, (3201, 1602, 3202, 11602, 1603), 1e-5)):
         for tile in tiles:
             st, _, stats = run_plate_gpu(pdpkg.pdhip, w, 0.0, 40, precision=prec, tile=tile)
@@ -679,14 +680,115 @@ def test_isolated_particles_and_tiny_families(pdpkg):
             assert stats.n_singular >= 2                # the pair's two bonds
 
 
-def test_batched_plates_equal_individual_runs(pdpkg):
-    say("batched plates (disjoint union) vs each plate alone")
+def test_batched_plates_fp64_match_oracle(pdpkg):
+    say("batched plates (disjoint union), FP64, no breaking, vs the oracle")
     pdh = pdpkg.pdhip
     wb = pdpkg.plates.plates_batched(count=4, seed=1, lo=24, hi=40)
     st, brb, _ = run_plate_gpu(pdh, wb, 0.0, 40)
     u, v, f, _ = run_plate_oracle(wb, 0.0, 40)
     _, nonbc, _ = H.parity_norms(st["v"], v, wb.code >= 0)
     assert nonbc < 1e-11
+
+
+def sub_plate(pl, wb, k):
+    """Plate k of a disjoint-union mesh as a mesh of its own (its particles and
+    bonds are contiguous; its two velocity groups become groups 0 and 1)."""
+    idx = np.nonzero(wb.plate_of == k)[0]
+    p0, p1 = int(idx[0]), int(idx[-1]) + 1
+    b0, b1 = int(wb.row[p0]), int(wb.row[p1])
+    code = wb.code[p0:p1].copy()
+    code[code >= 0] -= 2 * k
+    return pl.Workload(f"plate{k}", wb.X[2 * p0:2 * p1].copy(), wb.V[p0:p1].copy(),
+                       (wb.row[p0:p1 + 1] - b0).astype(np.int32), (wb.col[b0:b1] - p0).astype(np.int32),
+                       code, wb.v0[4 * k:4 * k + 4].copy(), wb.broken[b0:b1].copy(),
+                       horizon=wb.horizon, dx=wb.dx), (p0, p1, b0, b1)
+
+
+def test_batched_plates_breaking_match_oracle_and_single_runs(pdpkg):
+    """The batched configuration's arithmetic at a size the oracle runs: six
+    plates of 24..48 per side with their own notch, angle and edge velocity,
+    critical stretch 0.008 (8,800 bonds break in four of the six plates within
+    300 steps, none in the other two).  FP64 keeps the oracle's broken set with
+    velocities within 1e-10; with the tile fixed, each plate of the batched FP64
+    run is bit-identical to that plate run alone (the batch changes nothing but
+    the launch).  Mixed precision, stepped in lockstep with the FP64 run, has
+    the FP64 broken set after every step through step 200 and is within 1e-5
+    of it there, and ends with the oracle's broken set.  (Later a few bonds
+    whose stretch reaches s0 within the mixed-precision error of u break one
+    step apart; see test_batched_plates_mixed_moderate_breaking for a run
+    whose crack histories coincide step by step to the end.)"""
+    say("batched plates with breaking: 6 plates, s0 = 0.008, 300 steps, FP64 and mixed vs the oracle")
+    pdh = pdpkg.pdhip
+    wb = pdpkg.plates.plates_batched(count=6, seed=2, lo=24, hi=48)
+    s0, steps = 0.008, 300
+    u, v, f, br = run_plate_oracle(wb, s0, steps)
+    assert br.sum() > wb.broken.sum() + 5000
+    mask = wb.code >= 0
+    st, brg, _ = run_plate_gpu(pdh, wb, s0, steps, tile=1602)
+    np.testing.assert_array_equal(brg, br)
+    _, nonbc, _ = H.parity_norms(st["v"], v, mask)
+    assert nonbc < 1e-10, nonbc
+    for k in (2, 5):
+        w1, (p0, p1, b0, b1) = sub_plate(pdpkg.plates, wb, k)
+        s1, br1, _ = run_plate_gpu(pdh, w1, s0, steps, tile=1602)
+        np.testing.assert_array_equal(s1["v"], st["v"][2 * p0:2 * p1])
+        np.testing.assert_array_equal(s1["u"], st["u"][2 * p0:2 * p1])
+        np.testing.assert_array_equal(br1, brg[b0:b1])
+        assert br1.sum() > w1.broken.sum()
+    for tile in (11602, 1603):
+        c64 = wb.context(precision=0, s0=s0)
+        c64.set_option(pdh.OPT_TILE, 1602)
+        cm = wb.context(precision=1, s0=s0)
+        cm.set_option(pdh.OPT_TILE, tile)
+        differ = []
+        for step in range(1, steps + 1):
+            c64.step(1)
+            cm.step(1)
+            if not np.array_equal(c64.get_broken(), cm.get_broken()):
+                differ.append(step)
+            if step <= 200 and step % 50 == 0:
+                assert not differ, (tile, differ)
+                _, nonbc, _ = H.parity_norms(cm.get_state()["v"], c64.get_state()["v"], mask)
+                assert nonbc < 1e-5, (tile, step, nonbc)
+        bm = cm.get_broken()
+        c64.close()
+        cm.close()
+        say(f"  mixed tile {tile}: steps with differing broken sets {differ[:10]}{'...' if len(differ) > 10 else ''}")
+        np.testing.assert_array_equal(bm, br)
+
+
+def test_batched_plates_mixed_moderate_breaking(pdpkg):
+    """The same six plates at critical stretch 0.012 (several hundred bonds
+    break in the faster plates): the mixed-precision crack history equals the
+    FP64 one after every step, and the velocities stay within 1e-5 of the
+    oracle's to the end of the 300 steps, in the autotuned and in fixed tiles."""
+    say("batched plates, s0 = 0.012, 300 steps, mixed vs the oracle, step by step")
+    pdh = pdpkg.pdhip
+    wb = pdpkg.plates.plates_batched(count=6, seed=2, lo=24, hi=48)
+    s0, steps = 0.012, 300
+    u, v, f, br = run_plate_oracle(wb, s0, steps)
+    assert br.sum() > wb.broken.sum() + 300
+    mask = wb.code >= 0
+    for tune, tile in ((2, None), (0, 11602), (0, 1603), (0, 3202)):
+        c64 = wb.context(precision=0, s0=s0)
+        c64.set_option(pdh.OPT_TILE, 1602)
+        cm = wb.context(precision=1, s0=s0)
+        if tune:
+            cm.autotune(tune)
+        else:
+            cm.set_option(pdh.OPT_TILE, tile)
+        for step in range(steps):
+            c64.step(1)
+            cm.step(1)
+            np.testing.assert_array_equal(cm.get_broken(), c64.get_broken())
+        sm = cm.get_state()
+        bm = cm.get_broken()
+        c64.close()
+        cm.close()
+        _, nonbc, _ = H.parity_norms(sm["v"], v, mask)
+        say(f"  mixed tile {tile or 'autotuned'}: v error {nonbc:.3e}")
+        assert nonbc < 1e-5, (tile, nonbc)
+        np.testing.assert_array_equal(bm, br)
 
 
 def test_non_finite_state_is_reported(pdpkg, small_plate):

@@ -10,6 +10,10 @@ restatement is pinned instead by:
   in closed form, independent of orientation;
 * the limits of the damage law (0 at the threshold, 1 - (1 - A_t) kappa0 / eps
   far above it) and alpha_t = 1 / alpha_c = 0 for the strain split;
+* the tension law's rate constants at interior points: ln(1 - (1 - A_t)
+  kappa_r / eps - d) is linear in eps with slope -B_t and intercept ln A_t;
+* the compression law is unreachable (alpha_c = 0 for every strain), so A_c
+  and B_c never change d;
 * idempotence at a fixed strain, and the reference mode's running-threshold
   behaviour against the published (initial-threshold) law;
 * the affine patch test: u = eps0 X gives F_ik - I = eps0 for every bond, so
@@ -174,3 +178,54 @@ def test_broken_bonds_count_as_fully_damaged():
     np.testing.assert_array_equal(dmg, broken.astype(float))
     m = np.diff(row)
     np.testing.assert_allclose(status, np.add.reduceat(broken.astype(float), row[:-1]) / m)
+
+
+@pytest.mark.parametrize("mode,kappa_prev", [(O.DAMAGE_REFERENCE, 1.0), (O.DAMAGE_MAZARS, 1.0),
+                                             (O.DAMAGE_REFERENCE, 1.3), (O.DAMAGE_MAZARS, 1.3)])
+def test_tension_law_rate_constants_at_interior_points(mode, kappa_prev):
+    """Away from its limits the tension law pins A_t and B_t: for a loading
+    past the threshold kappa_r used in the law (kappa0 in the published law,
+    the running threshold in the reference's, src/main_seq.cpp:579-584),
+    ln(1 - (1 - A_t) kappa_r / eps - d) = ln A_t - B_t (eps - kappa_r) is a
+    straight line in eps with slope -B_t.  Sampled where the exponential is
+    between 0.3 and 0.9 of A_t, so a swapped B_t / B_c (slope -2150) or a wrong
+    A_t fails."""
+    dp = O.damage_params(mode=mode)
+    kap = kappa_prev * KAPPA0
+    kr = KAPPA0 if mode == O.DAMAGE_MAZARS else kap
+    eps = kap * np.linspace(1.02, 1.6, 12)
+    y = []
+    for e in eps:
+        upd, k, d = O.mazars_update(dp, e, 0.0, 0.0, kap)
+        assert upd and k == e
+        y.append(np.log(1.0 - (1.0 - dp.At) * kr / e - d))
+    slope, icpt = np.polyfit(eps - kr, np.array(y), 1)
+    assert slope == pytest.approx(-dp.Bt, rel=1e-9)
+    assert icpt == pytest.approx(np.log(dp.At), rel=1e-9)
+    # a planted swap of the tension and compression rates moves the line
+    bad = O.damage_params(mode=mode, Bt=dp.Bc, Bc=dp.Bt)
+    yb = [np.log(1.0 - (1.0 - bad.At) * kr / e - O.mazars_update(bad, e, 0.0, 0.0, kap)[2]) for e in eps]
+    assert np.polyfit(eps - kr, np.array(yb), 1)[0] == pytest.approx(-dp.Bc, rel=1e-9)
+
+
+def test_compression_law_is_unreachable():
+    """strain_c holds the non-positive principal parts (src/main_seq.cpp:556-562),
+    so its eigenvalues are never > 0 and alpha_c stays 0 (:571-577): the
+    compression constants A_c, B_c never enter d.  For tension-compression
+    states alpha_t = 1 (the ascending spectra pair l2 with 0 and 0 with l1),
+    so d is the tension law of the positive part alone."""
+    rng = np.random.default_rng(3)
+    dp = O.damage_params()
+    odd = O.damage_params(Ac=0.05, Bc=7.0e5)
+    for _ in range(300):
+        lam = np.sort(rng.uniform(-6, 6, size=2)) * KAPPA0
+        R = rot(rng.uniform(0, np.pi))
+        eps = R @ np.diag(lam) @ R.T
+        a = O.mazars_update(dp, eps[0, 0], eps[1, 1], eps[0, 1], KAPPA0)
+        b = O.mazars_update(odd, eps[0, 0], eps[1, 1], eps[0, 1], KAPPA0)
+        assert a == b
+        eq = np.sqrt(np.sum(np.maximum(lam, 0.0) ** 2))
+        if eq > KAPPA0:
+            assert a[0]
+            d_t = 1.0 - KAPPA0 * (1 - dp.At) / eq - dp.At * np.exp(-dp.Bt * (eq - KAPPA0))
+            assert a[2] == pytest.approx(d_t, rel=1e-12, abs=1e-15)
