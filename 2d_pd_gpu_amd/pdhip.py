@@ -55,6 +55,7 @@ EXPORTS = (
     "pd_set_damage", "pd_compute_damage", "pd_get_damage", "pd_set_damage_state",
     "pd_read_mesh", "pd_write_velocity", "pd_write_positions", "pd_write_gathered",
     "pd_create_from_points", "pd_get_families", "pd_save_checkpoint", "pd_load_checkpoint",
+    "pd_compute_velocity_static",
 )
 
 
@@ -160,6 +161,7 @@ def lib():
         "pd_write_velocity": (I, [ctx, C.c_char_p, I]),
         "pd_write_positions": (I, [ctx, C.c_char_p, P, I]),
         "pd_write_gathered": (I, [ctx, C.c_char_p, P, I]),
+        "pd_compute_velocity_static": (I, [C.POINTER(Params), I, C.c_long, I] + [P] * 9 + [I, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

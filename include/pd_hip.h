@@ -284,6 +284,34 @@ int pd_load_checkpoint(pd_context* ctx, const char* path);
  * traffic to scale HBM counter readings (FETCH_SIZE / WRITE_SIZE) against. */
 int pd_probe_copy(long bytes, int iters, double* gbps);
 
+/* ---- allocation-free force evaluation (no counterpart in the reference) -----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  One step of
+ * pd_compute_velocity (bond breaking when s0 > 0, force density f, velocity
+ * update v += 1/2 (a + f/rho) dt, and a = f/rho when write_acceleration),
+ * PD_FP64 only, on a mesh and state held entirely in the caller's device
+ * buffers, in the caller's particle order:
+ *   X, u, v, a, f   double[2n] (xy interleaved, 16-byte aligned); V double[n];
+ *   row_ptr int[n+1], col int[nb] (symmetric families, as pd_create takes);
+ *   broken  uint8[nb], in/out (bonds that break are set to 1).
+ * nb = row_ptr[n] and max_family (the largest family) are passed by the host.
+ * Every intermediate (reverse bonds, K^-1, the force terms) lives in a fixed
+ * per-GPU workspace inside the library (about 24 MiB, held until the process
+ * ends), so the call allocates nothing; it works for n <= PD_STATIC_MAX_PARTICLES
+ * and nb <= PD_STATIC_MAX_BONDS and rejects larger meshes.  Work is enqueued on
+ * `stream` (hipStream_t, NULL = the null stream).  Outside stream capture the
+ * call first checks the families on the device (symmetric, in range, no
+ * self bond, no repeats) and waits for that check; under capture it does not
+ * wait, and a mesh the call has not accepted before must not be passed.
+ * Calls that overlap on one GPU share its workspace and race: serialise them. */
+#define PD_STATIC_MAX_PARTICLES 131072
+#define PD_STATIC_MAX_BONDS 524288
+int pd_compute_velocity_static(const pd_params* p, int n, long nb, int max_family,
+                               const double* X, const double* V, const int* row_ptr,
+                               const int* col, unsigned char* broken, const double* u,
+                               double* v, double* a, double* f, int write_acceleration,
+                               void* stream);
+
 /* ---- setup helpers (host side; no counterpart in the hot path) -------------- */
 /* Families with the reference cutoff |xi| < 1.01*dx*horizon
  * (src/SetupParticleSystem.cpp:113) from a cell list, neighbours in ascending
