@@ -22,6 +22,8 @@
 // kernel then reads one contiguous row per particle.
 #pragma once
 
+#include <type_traits>
+
 #include "pd_internal.h"
 
 namespace pd {
@@ -490,6 +492,7 @@ template <typename R, int BPL> struct FamLoad {
     double Vj[BPL];
     double kap[BPL], dm[BPL];  // damage threshold and value (damage kernels)
     unsigned char ff;          // the family's guard flag (mixed precision)
+    unsigned char rk[BPL];     // rank of the bond's length in its family (MW contractions)
     static constexpr bool GUARD = sizeof(R) == 4;
 };
 
@@ -500,7 +503,7 @@ __device__ __forceinline__ void stage_rows(const BondArgs& a, FL& L, int lane, i
     if constexpr (FL::GUARD) L.ff = a.famflag[L.F.famok ? L.F.fam : 0];
 }
 
-template <typename R, int LPF, int BPL, int DMG = 0, int FIX = 0>
+template <typename R, int LPF, int BPL, int DMG = 0, int FIX = 0, int RK = 0>
 __device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<R, BPL>& L)
 {
     family_extent(L.F);
@@ -527,6 +530,7 @@ __device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<R, BPL>& L
             L.kap[s] = a.kappa[b];
             L.dm[s] = a.dmg[b];
         }
+        if constexpr (RK) L.rk[s] = a.rank[b];
     }
 }
 
@@ -578,8 +582,217 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 //    component (lane >> 4) of D_kb.  Families are taken one after the other and
 //    the results go back to the bond-per-lane layout through LDS.
 template <typename R, int MF> struct Contraction {
-    static constexpr bool MF32 = MF && sizeof(R) == 4;
-    static constexpr bool MF64 = MF && sizeof(R) == 8;
+    static constexpr bool MF32 = MF == 1 && sizeof(R) == 4;
+    static constexpr bool MF64 = MF == 1 && sizeof(R) == 8;
+    static constexpr bool MW = MF == 2;
+};
+
+// ---------------------------------------------------------------------------
+// Matrix-core weights (MF = 2, "MW"): the weight of a pair comes from a rank-3
+// product on the matrix cores instead of ten VALU operations.
+//
+// With p = exp(A l / n2), q = 1 / p and h = 1/2 + 1/2 e_k.e_l,
+//   w(k,l) = exp(-A |l_k - l_l|) h^n2 = (g / 2)^n2,
+//   g = U_k . L_l (l_l >= l_k)  or  L_k . U_l (l_l <= l_k),
+// where U = p (1, e_x, e_y) and L = q (1, e_x, e_y): the radial factor folds
+// into the base of the power, and which of the two forms applies follows from
+// the order of the lengths.  Each family's bonds are given their length rank
+// at set-up (rank[]), and their records are written to LDS in rank order, so in
+// a 16 x 16 block of (bond, neighbour) ranks every pair has the same form,
+// except on the diagonal blocks, which take the smaller of the two (it is the
+// one that applies).  Per 16 x 16 block:
+//   G^T = L-or-U(neighbours) x U-or-L(bonds)^T   one v_mfma_{f32,f64}_16x16x4
+//   w   = G^10 (the 1/2^n2 is folded into the P / Q records)    4 VALU per pair
+//   D  += w P                      FP32: four v_mfma_f32_4x4x1 (16 blocks)
+//                                  FP64: four v_mfma_f64_4x4x4 (4 blocks)
+// Output layouts (checked on the device): 16x16x4 f32, lane l register r holds
+// C[4 (l / 16) + r][l % 16]; 16x16x4 f64, C[l / 16 + 4 r][l % 16].  A family's
+// blocks are taken one after another by the whole wave; the results go back
+// to the records' P slots, from where each lane reads its bonds' sums.
+// ---------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <typename R, int LPF, int BPL> struct MWL {
+    static constexpr int FS = LPF * BPL;
+    static constexpr int FPW = 64 / LPF;
+    static constexpr int NB = FS / 16;
+    // slots per component row: an odd multiple of 16, so that the two halves
+    // of a wave (components c and c + 1) read distinct banks
+    static constexpr int FSP = FS % 32 == 0 ? FS + 16 : FS;
+    // per family 10 rows: U (3), L (3), P / Q (4); the fourth component of
+    // the rank-3 Gram products is a zero the lanes supply themselves
+    static constexpr int ROWS = 10, PROW = 6;
+    static constexpr int DOUBLES = (int)((FPW * ROWS * FSP * sizeof(R) + 7) / 8);
+    static __device__ __forceinline__ int at(int f, int c, int slot) { return (f * ROWS + c) * FSP + slot; }
+};
+
+template <typename R, int N2_IS_10>
+__device__ __forceinline__ void mw_geometry(const BondArgs& a, const BondSlot& S, R (&U)[3], R (&Lq)[3])
+{
+    if constexpr (sizeof(R) == 8) {
+        const double l2 = S.x0 * S.x0 + S.x1 * S.x1;
+        const double il = S.act ? rsqrt64(l2) : 0.0;
+        const double len = l2 * il;
+        const double ex = S.x0 * il, ey = S.x1 * il;
+        const double Ar = N2_IS_10 ? a.A * 0.1 : a.A / a.n2;
+        const double p = S.act ? exp(Ar * len) : 0.0;
+        const double q = S.act ? rcp64(p) : 0.0;
+        U[0] = p; U[1] = p * ex; U[2] = p * ey;
+        Lq[0] = q; Lq[1] = q * ex; Lq[2] = q * ey;
+    } else {
+        const float x0 = float(S.x0), x1 = float(S.x1);
+        const float l2 = x0 * x0 + x1 * x1;
+        const float il = S.act ? __builtin_amdgcn_rsqf(l2) : 0.0f;
+        const float len = l2 * il;
+        const float ex = x0 * il, ey = x1 * il;
+        const float Ar = N2_IS_10 ? float(a.A * 0.1) : float(a.A / a.n2);
+        const float p = S.act ? __expf(Ar * len) : 0.0f;
+        const float q = S.act ? __builtin_amdgcn_rcpf(p) : 0.0f;
+        U[0] = p; U[1] = p * ex; U[2] = p * ey;
+        Lq[0] = q; Lq[1] = q * ex; Lq[2] = q * ey;
+    }
+}
+
+// G^T block on the matrix cores: lane l gives A = component l/16 of its
+// neighbour record, B = component l/16 of its bond record.
+__device__ __forceinline__ v4f mw_gram(float a, float b)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, v4f{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+}
+__device__ __forceinline__ v4d mw_gram(double a, double b)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, v4d{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+}
+
+template <int N2_IS_10, typename R>
+__device__ __forceinline__ R mw_pow(R g, R n2)
+{
+    if constexpr (N2_IS_10) {
+        const R g2 = g * g;
+        const R g4 = g2 * g2;
+        const R g8 = g4 * g4;
+        return g8 * g2;
+    } else {
+        return rpow(rmax(g, R(0)), n2);
+    }
+}
+
+// One contraction X_k = sum_l w(k,l) Y_l for every family of the wave, Y in
+// the P rows of the MW records; X replaces Y there.  Per family the neighbour
+// blocks s2 are the outer loop: the U and L rows of every block are read
+// once, the 16 x 16 Gram blocks of all bond blocks s1 against s2 are
+// independent matrix-core products, and each s1 keeps its own accumulator, so
+// the MFMA chains of different s1 interleave.  Every block of the tile is
+// evaluated (slots past a family's end hold zero records and add nothing), so
+// the loop is straight-line code the scheduler can interleave.
+template <typename R, int LPF, int BPL, int N2_IS_10, typename HA, typename HB>
+__device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& hook_a, const HB& hook_b)
+{
+    typedef MWL<R, LPF, BPL> M;
+    constexpr int FSP = M::FSP;
+    constexpr int NB = M::NB;
+    constexpr int PR = M::PROW;
+    // families taken together (their chains interleave)
+    constexpr int FG = M::FPW >= 2 ? 2 : 1;
+    const int lo = lane & 15, hi = lane >> 4, lc = lane & 3;
+    const int hc = hi < 3 ? hi : 2;   // lanes of component 3 read a valid row, then supply 0
+    typedef typename std::conditional<sizeof(R) == 4, v4f, double>::type Acc;
+    typedef decltype(mw_gram(R(0), R(0))) G4;
+#pragma unroll
+    for (int f0 = 0; f0 < M::FPW; f0 += FG) {
+        R* base[FG];
+        R U[FG][NB], Lq[FG][NB];
+        Acc acc[FG][NB];
+#pragma unroll
+        for (int e = 0; e < FG; ++e) {
+            base[e] = lds + M::at(f0 + e, 0, 0);
+            // component hi of the U and L records of slot 16 s + lo
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const R u = base[e][hc * FSP + 16 * b + lo];
+                const R l = base[e][(3 + hc) * FSP + 16 * b + lo];
+                U[e][b] = hi < 3 ? u : R(0);
+                Lq[e][b] = hi < 3 ? l : R(0);
+                if constexpr (sizeof(R) == 4) acc[e][b] = v4f{0.f, 0.f, 0.f, 0.f};
+                else acc[e][b] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < NB; ++s2) {
+            R w[FG][NB][4];
+#pragma unroll
+            for (int e = 0; e < FG; ++e)
+#pragma unroll
+                for (int s1 = 0; s1 < NB; ++s1) {
+                    G4 g;
+                    if (s2 > s1) {
+                        g = mw_gram(Lq[e][s2], U[e][s1]);
+                    } else if (s2 < s1) {
+                        g = mw_gram(U[e][s2], Lq[e][s1]);
+                    } else {
+                        const G4 gu = mw_gram(Lq[e][s2], U[e][s1]), gl = mw_gram(U[e][s2], Lq[e][s1]);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) g[r] = gu[r] < gl[r] ? gu[r] : gl[r];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) w[e][s1][r] = mw_pow<N2_IS_10>(R(g[r]), n2);
+                }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int e = 0; e < FG; ++e) {
+                    R pa;
+                    if constexpr (sizeof(R) == 4) pa = base[e][(PR + lc) * FSP + 16 * s2 + 4 * hi + r];
+                    else pa = base[e][(PR + lc) * FSP + 16 * s2 + 4 * r + hi];
+#pragma unroll
+                    for (int s1 = 0; s1 < NB; ++s1) {
+                        if constexpr (sizeof(R) == 4)
+                            acc[e][s1] = __builtin_amdgcn_mfma_f32_4x4x1f32(pa, w[e][s1][r], acc[e][s1], 0, 0, 0);
+                        else
+                            acc[e][s1] = __builtin_amdgcn_mfma_f64_4x4x4f64(pa, w[e][s1][r], acc[e][s1], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if constexpr (sizeof(R) == 4) {
+            // partial sums of lanes l, l + 16, l + 32, l + 48 (the four
+            // neighbour groups of each 4x4x1 block row)
+#pragma unroll
+            for (int e = 0; e < FG; ++e)
+#pragma unroll
+                for (int s1 = 0; s1 < NB; ++s1)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        acc[e][s1][c] += __shfl_xor(acc[e][s1][c], 16, 64);
+                        acc[e][s1][c] += __shfl_xor(acc[e][s1][c], 32, 64);
+                    }
+        }
+        wave_sync();
+#pragma unroll
+        for (int e = 0; e < FG; ++e)
+#pragma unroll
+            for (int s1 = 0; s1 < NB; ++s1) {
+                if constexpr (sizeof(R) == 4) {
+                    // lane l < 16 holds the four components of slot 16 s1 + l
+                    if (lane < 16) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) base[e][(PR + c) * FSP + 16 * s1 + lane] = acc[e][s1][c];
+                    }
+                } else {
+                    // lane l holds component l / 16 of slot 16 s1 + l % 16
+                    base[e][(PR + hi) * FSP + 16 * s1 + lo] = acc[e][s1];
+                }
+            }
+        if (f0 == 0) hook_a();
+    }
+    hook_b();
+}
+
+// LDS per wave of a bond-kernel variant, in doubles.
+template <typename R, int LPF, int BPL, int MF> struct WaveLds {
+    static constexpr int DOUBLES = MF == 2 && MWL<R, LPF, BPL>::DOUBLES > Tile<LPF, BPL>::LDS_DOUBLES
+                                       ? MWL<R, LPF, BPL>::DOUBLES
+                                       : Tile<LPF, BPL>::LDS_DOUBLES;
 };
 
 // One explicit force evaluation of a family group: bond breaking, K
@@ -599,8 +812,9 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     finish_loads(L);
     constexpr bool MF32 = Contraction<R, MF>::MF32;
     constexpr bool MF64 = Contraction<R, MF>::MF64;
-    static_assert(!MF64 || (Tile<LPF, BPL>::FS % 16 == 0),
-                  "FP64 matrix-core contraction: families of a multiple of 16 slots");
+    constexpr bool MW = Contraction<R, MF>::MW;
+    static_assert(!(MF64 || MW) || (Tile<LPF, BPL>::FS % 16 == 0),
+                  "matrix-core contractions: families of a multiple of 16 slots");
     typedef typename Vec2<R>::T R2;
     constexpr int FS = Tile<LPF, BPL>::FS;
     constexpr int FPW = Tile<LPF, BPL>::FPW;
@@ -673,6 +887,34 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     constexpr bool COMPACT = sizeof(R) == 8 && !MF && BPL == 1;
     R* rec = reinterpret_cast<R*>(__builtin_assume_aligned(rec64, 16));
     Geo<R> g[BPL];
+    // MW records: rank slot of each of the lane's bonds (inactive slots keep
+    // their own position, past the family's ranks), and the 2^-n2 of (g/2)^n2
+    int mws[BPL];
+    const R mw_scale = N2_IS_10 ? R(1.0 / 1024.0) : R(exp2(-a.n2));
+    if constexpr (MW) {
+        typedef MWL<R, LPF, BPL> M;
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) {
+            mws[s] = S[s].act ? int(L.rk[s]) : S[s].k;
+            R U[3], Lq[3];
+            mw_geometry<R, N2_IS_10>(a, S[s], U, Lq);
+            R* r0 = rec + M::at(F.slot, 0, mws[s]);
+            r0[0] = U[0];
+            r0[M::FSP] = U[1];
+            r0[2 * M::FSP] = U[2];
+            r0[3 * M::FSP] = Lq[0];
+            r0[4 * M::FSP] = Lq[1];
+            r0[5 * M::FSP] = Lq[2];
+            const R x0r = R(S[s].x0), x1r = R(S[s].x1);
+            const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) * mw_scale : R(0);
+            const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) * mw_scale : R(0);
+            r0[6 * M::FSP] = Vd0 * x0r;
+            r0[7 * M::FSP] = Vd0 * x1r;
+            r0[8 * M::FSP] = Vd1 * x0r;
+            r0[9 * M::FSP] = Vd1 * x1r;
+        }
+    }
+    if constexpr (!MW) {
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         g[s] = GeoOf<R>::get(a, S[s]);
@@ -687,6 +929,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) : R(0);
             rec_put4<FPW>(rec, k, F.slot, 1, Vd0 * x0r, Vd0 * x1r, Vd1 * x0r, Vd1 * x1r);
         }
+    }
     }
     // records past a family's end are zero (inactive slots), so the loops may
     // run to the wave's largest family rounded up to 4 without guards
@@ -717,8 +960,8 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     // are evaluated again there (FP64 two-bonds-per-lane: one row of 32 doubles
     // fewer, measured -2.4 %; three bonds per lane, rows of 48: one row in
     // FP32, none in FP64)
-    constexpr int NC = (MF64 || (!MF && FS >= 64)) ? 0
-                       : BPL == 3                    ? (sizeof(R) == 8 ? 0 : 1)
+    constexpr int NC = (MF64 || MW || (!MF && FS >= 64)) ? 0
+                       : BPL == 3                    ? (sizeof(R) == 8 ? 1 : 3)
                        : (sizeof(R) == 8 && BPL == 2) ? 1
                                                       : BPL;
     constexpr bool RW = NC < BPL;
@@ -734,7 +977,14 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
         D[s][0] = D[s][1] = D[s][2] = D[s][3] = R(0);
         Dv[s] = v4f{0.f, 0.f, 0.f, 0.f};
     }
-    if constexpr (MF64) {
+    if constexpr (MW) {
+        mw_contract<R, LPF, BPL, N2_IS_10>(rec, lane, n2r, hook_a, hook_b);
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) D[s][c] = rec[MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW + c, mws[s])];
+    } else if constexpr (MF64) {
         double Dacc[WF][NS];
 #pragma unroll
         for (int f = 0; f < WF; ++f) {
@@ -908,8 +1158,15 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     wave_sync();
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
-        const int k = S[s].k;
-        rec_put4<FPW>(rec, k, F.slot, 1, Q[s][0], Q[s][1], Q[s][2], Q[s][3]);
+        if constexpr (MW) {
+            constexpr int FSPm = MWL<R, LPF, BPL>::FSP;
+            R* r8 = rec + MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW, mws[s]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) r8[c * FSPm] = Q[s][c] * mw_scale;
+        } else {
+            const int k = S[s].k;
+            rec_put4<FPW>(rec, k, F.slot, 1, Q[s][0], Q[s][1], Q[s][2], Q[s][3]);
+        }
     }
     wave_sync();
 
@@ -917,7 +1174,14 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     R Sm[BPL][4];
 #pragma unroll
     for (int s = 0; s < BPL; ++s) Sm[s][0] = Sm[s][1] = Sm[s][2] = Sm[s][3] = R(0);
-    if constexpr (MF64) {
+    if constexpr (MW) {
+        mw_contract<R, LPF, BPL, N2_IS_10>(rec, lane, n2r, NoHook(), NoHook());
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW + c, mws[s])];
+    } else if constexpr (MF64) {
         double Sacc[WF][NS];
 #pragma unroll
         for (int f = 0; f < WF; ++f) {
@@ -1094,9 +1358,9 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
     FamLoad<R, BPL> L;
     stage_rows<LPF>(a, L, lane, blk * wpb + wib);
-    stage_cols<R, LPF, BPL, DMG>(a, L);
+    stage_cols<R, LPF, BPL, DMG, 0, MF == 2>(a, L);
     stage_bonds<R, LPF, BPL>(a, L);
-    bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * Tile<LPF, BPL>::LDS_DOUBLES, lane,
+    bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * WaveLds<R, LPF, BPL, MF>::DOUBLES, lane,
                                               NoHook(), NoHook());
 }
 
@@ -1118,12 +1382,12 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
     const int per = ngroups / (int)gridDim.x, rem = ngroups % (int)gridDim.x;
     const int g0 = blk * per + (blk < rem ? blk : rem);
     const int g1 = g0 + per + (blk < rem ? 1 : 0);
-    double* rec64 = smem + wib * Tile<LPF, BPL>::LDS_DOUBLES;
+    double* rec64 = smem + wib * WaveLds<R, LPF, BPL, MF>::DOUBLES;
     int g = g0 + wib;
     if (g >= g1) return;
     FamLoad<R, BPL> cur, nxt;
     stage_rows<LPF>(a, cur, lane, g);
-    stage_cols<R, LPF, BPL, DMG>(a, cur);
+    stage_cols<R, LPF, BPL, DMG, 0, MF == 2>(a, cur);
     stage_bonds<R, LPF, BPL>(a, cur);
     for (;;) {
         const int gn = g + wpb;
@@ -1135,7 +1399,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
         stage_rows<LPF>(a, nxt, lane, gp);
         bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(
             a, cur, rec64, lane,
-            [&]() { stage_cols<R, LPF, BPL, DMG>(a, nxt); },
+            [&]() { stage_cols<R, LPF, BPL, DMG, 0, MF == 2>(a, nxt); },
             [&]() { stage_bonds<R, LPF, BPL>(a, nxt); });
         if (!more) break;
         cur = nxt;

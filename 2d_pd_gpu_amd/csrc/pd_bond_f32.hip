@@ -20,6 +20,7 @@ static hipError_t launch_f32_mf(const BondArgs& a, int lpf, int bpl, int n2_is_1
 hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s)
 {
+    if (mf == 2) return hipErrorInvalidValue;   // matrix-core weights: FP64 only
     if (dmg) return mf ? launch_f32_mf<1, 1>(a, lpf, bpl, n2_is_10, wpb, pgrid, s)
                        : launch_f32_mf<0, 1>(a, lpf, bpl, n2_is_10, wpb, pgrid, s);
     return mf ? launch_f32_mf<1, 0>(a, lpf, bpl, n2_is_10, wpb, pgrid, s)

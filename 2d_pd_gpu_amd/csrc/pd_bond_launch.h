@@ -12,7 +12,7 @@ static hipError_t launch_bond_t(const BondArgs& a, int wpb, int pgrid, hipStream
     const long groups = ((long)a.n + Tile<LPF, BPL>::FPW - 1) / Tile<LPF, BPL>::FPW;
     const int grid = (int)((groups + wpb - 1) / wpb);
     if (grid == 0) return hipSuccess;
-    const size_t lds = (size_t)wpb * Tile<LPF, BPL>::LDS_DOUBLES * sizeof(double);
+    const size_t lds = (size_t)wpb * WaveLds<R, LPF, BPL, MF>::DOUBLES * sizeof(double);
     if (pgrid > 0 && pgrid < grid)
         hipLaunchKernelGGL((k_bond_p<R, LPF, BPL, N2, MF, DMG>), dim3(pgrid), dim3(64 * wpb), lds, s, a);
     else

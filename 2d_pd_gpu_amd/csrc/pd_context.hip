@@ -29,7 +29,7 @@ static void free_all(pd_context* c)
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
     void* ptrs[] = {c->X, c->u, c->v, c->a, c->f, c->fown, c->v0, c->V, c->E, c->pf1, c->kinv,
-                    c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->W, c->counters,
+                    c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->rank, c->W, c->counters,
                     c->kappa, c->dmg, c->dstat, c->famflag, c->famlist, c->famcnt};
     for (void* q : ptrs)
         if (q) hipFree(q);
@@ -50,6 +50,7 @@ static BondArgs bond_args(pd_context* c)
     a.row = c->row;
     a.col = c->col;
     a.rev = c->rev;
+    a.rank = c->rank;
     a.u = c->u;
     a.broken = c->broken;
     a.kinv = c->kinv;
@@ -253,6 +254,7 @@ static int finish_create(pd_context* c)
     if ((rc = dalloc(c, &c->u, n)) || (rc = dalloc(c, &c->v, n)) || (rc = dalloc(c, &c->a, n)) ||
         (rc = dalloc(c, &c->f, n)) || (rc = dalloc(c, &c->fown, n)) ||
         (rc = dalloc(c, (char**)&c->kinv, kinv_bytes(c))) || (rc = dalloc(c, &c->broken, nb)) ||
+        (rc = dalloc(c, &c->rank, nb)) ||
         (rc = dalloc(c, &c->pf1, 1)) || (rc = dalloc(c, (char**)&c->W, wbytes * (size_t)(nb > 0 ? nb : 1))) ||
         (rc = dalloc(c, &c->counters, 4)) || (rc = dalloc(c, &c->bc, n)) || (rc = dalloc(c, &c->v0, 1)))
         return rc;
@@ -291,6 +293,8 @@ static int finish_create(pd_context* c)
         std::vector<int> code(n, PD_BC_FREE);
         e = e != hipSuccess ? e : hipMemcpy(c->bc, code.data(), sizeof(int) * n, hipMemcpyHostToDevice);
     }
+    if (nb == 0) e = e != hipSuccess ? e : hipMemset(c->rank, 0, 1);
+    e = e != hipSuccess ? e : launch_ranks(n, c->row, c->col, c->X, c->rank, 0);
     e = e != hipSuccess ? e : hipDeviceSynchronize();
     if (e != hipSuccess) return fail(c, PD_ERR_HIP, "initialisation failed: %s", hipGetErrorString(e));
     return run_shape(c);
@@ -570,7 +574,9 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
             lpf = c->maxM <= 32 ? 32 : 64;
             bpl = 1;
         }
-        if (value < 0 || mf > 1 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
+        if (mf == 2 && !(c->p.n2 > 0))
+            return fail(c, PD_ERR_INVALID, "matrix-core weights need an angular exponent n2 > 0");
+        if (value < 0 || mf > 2 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
             lpf * bpl < c->maxM)
             return fail(c, PD_ERR_INVALID, "tile %ld (lanes %d x bonds %d, matrix cores %d) unsupported for "
                         "this precision or smaller than the largest family (%d)", value, lpf, bpl, mf, c->maxM);
@@ -1163,9 +1169,10 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     std::vector<Var> vars;
     const int mixed = c->p.precision == PD_MIXED;
     const int tiles[5][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}, {16, 3}};
-    for (int mf : {0, 1})
+    for (int mf : {0, 1, 2})
         for (const auto& tl : tiles) {
             if (!tile_supported(mixed, tl[0], tl[1], mf) || tl[0] * tl[1] < c->maxM) continue;
+            if (mf == 2 && !(c->p.n2 > 0)) continue;
             for (int w : {1, 2, 4, 8})
                 for (int x : {1, 8})
                     for (int pz : {0, 1, 2, 4})

@@ -18,6 +18,7 @@ struct BondArgs {
     const int* row;           // CSR row pointer, n+1
     const int* col;           // neighbour of each bond
     const int* rev;           // reverse bond of each bond
+    const unsigned char* rank; // per bond: rank of |xi| in its family (ties by position)
     const double2* u;         // displacement y - X
     unsigned char* broken;    // per bond, 1 = broken (irreversible)
     void* kinv;               // per bond K_ik^-1 (xx, xy, yy), assembled and inverted in
@@ -82,7 +83,8 @@ struct PosArgs {
 // (16,3)} and, for mixed precision, (32,2); a family may hold up to
 // lpf * bpl bonds.
 // mf = 1: family contractions on the matrix cores (every tile except FP64
-// (32,2), which does not exist).
+// (32,2), which does not exist).  mf = 2 (FP64 only): matrix-core weights and
+// contractions (tiles (32,1), (64,1), (16,3); n2 > 0).
 bool tile_supported(int mixed, int lpf, int bpl, int mf);
 // pgrid > 0: persistent launch with pgrid workgroups (each walks a contiguous
 // range of family groups, prefetching the next group's loads).
@@ -102,6 +104,10 @@ hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double
                                    double2* a, hipStream_t s);
 hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
+// rank[b] = rank of bond b's reference length |X_j - X_i| within its family
+// (ties by position): the record order of the matrix-core-weight kernels.
+hipError_t launch_ranks(int n, const int* row, const int* col, const double2* X, unsigned char* rank,
+                        hipStream_t s);
 hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, int dmg, int grid, hipStream_t s);
 hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32, const unsigned char* broken,
                                        double cmin, unsigned int* count, hipStream_t s);

@@ -186,6 +186,29 @@ __global__ void __launch_bounds__(256) k_count_ill_conditioned(long nb, const KT
     if ((threadIdx.x & 63) == 0 && mine) atomicAdd(count, mine);
 }
 
+// Rank of each bond's reference length in its family (squared lengths in
+// FP64, ties broken by position), one thread per family.
+__global__ void __launch_bounds__(256) k_ranks(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                               const double2* __restrict__ X, unsigned char* __restrict__ rank)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = row[i], m = row[i + 1] - b0;
+    const double2 xi = X[i];
+    double len2[64];
+    for (int e = 0; e < m && e < 64; ++e) {
+        const double2 xj = X[col[b0 + e]];
+        const double d0 = xj.x - xi.x, d1 = xj.y - xi.y;
+        len2[e] = d0 * d0 + d1 * d1;
+    }
+    for (int e = 0; e < m && e < 64; ++e) {
+        int r = 0;
+        for (int f = 0; f < m && f < 64; ++f)
+            r += (len2[f] < len2[e] || (len2[f] == len2[e] && f < e)) ? 1 : 0;
+        rank[b0 + e] = (unsigned char)r;
+    }
+}
+
 __global__ void k_fill_u32(unsigned int* p, unsigned int v, int count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -208,6 +231,7 @@ static inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
 bool tile_supported(int mixed, int lpf, int bpl, int mf)
 {
+    if (mf == 2) return !mixed && ((bpl == 1 && (lpf == 32 || lpf == 64)) || (lpf == 16 && bpl == 3));
     if (mf && !mixed) return (bpl == 1 && (lpf == 32 || lpf == 64)) || (lpf == 16 && (bpl == 2 || bpl == 3));
     if (bpl == 1) return lpf == 32 || lpf == 64;
     if (bpl == 2) return lpf == 16 || (lpf == 32 && mixed);
@@ -316,6 +340,14 @@ hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32
     else
         hipLaunchKernelGGL(k_count_ill_conditioned<double>, dim3(grid), dim3(256), 0, s, nb,
                            static_cast<const double*>(kinv), broken, r, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_ranks(int n, const int* row, const int* col, const double2* X, unsigned char* rank,
+                        hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ranks, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, row, col, X, rank);
     return hipGetLastError();
 }
 
