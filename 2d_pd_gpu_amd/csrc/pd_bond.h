@@ -293,7 +293,7 @@ __device__ __forceinline__ double rcp64(double x)
     return r;
 }
 
-__device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondSlot& S)
+__device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondSlot& S, double Ae)
 {
     Geo<double> g;
     const double l2 = S.x0 * S.x0 + S.x1 * S.x1;
@@ -304,13 +304,13 @@ __device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondS
     g.ey = S.x1 * il;
     g.hx = 0.5 * g.ex;
     g.hy = 0.5 * g.ey;
-    g.pe = S.act ? exp(a.A * len) : 0.0;
+    g.pe = S.act ? exp(Ae * len) : 0.0;
     g.qe = S.act ? rcp64(g.pe) : 0.0;
     return g;
 }
 
 // FP32 geometry for the mixed-precision contractions (hardware rsq / exp / rcp).
-__device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSlot& S)
+__device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSlot& S, double Ae)
 {
     Geo<float> g;
     const float x0 = float(S.x0), x1 = float(S.x1);
@@ -322,7 +322,7 @@ __device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSl
     g.ey = x1 * il;
     g.hx = 0.5f * g.ex;
     g.hy = 0.5f * g.ey;
-    const float Al = float(a.A) * len;
+    const float Al = float(Ae) * len;
     g.pe = S.act ? __expf(Al) : 0.0f;
     g.qe = S.act ? __builtin_amdgcn_rcpf(g.pe) : 0.0f;
     return g;
@@ -330,10 +330,16 @@ __device__ __forceinline__ Geo<float> geometry32(const BondArgs& a, const BondSl
 
 template <typename R> struct GeoOf;
 template <> struct GeoOf<double> {
-    static __device__ __forceinline__ Geo<double> get(const BondArgs& a, const BondSlot& S) { return geometry64(a, S); }
+    static __device__ __forceinline__ Geo<double> get(const BondArgs& a, const BondSlot& S, double Ae)
+    {
+        return geometry64(a, S, Ae);
+    }
 };
 template <> struct GeoOf<float> {
-    static __device__ __forceinline__ Geo<float> get(const BondArgs& a, const BondSlot& S) { return geometry32(a, S); }
+    static __device__ __forceinline__ Geo<float> get(const BondArgs& a, const BondSlot& S, double Ae)
+    {
+        return geometry32(a, S, Ae);
+    }
 };
 
 // FP64 assembly of K_ik over the alive bonds of each family in the wave, and
@@ -449,7 +455,7 @@ __global__ void __launch_bounds__(512) k_shape(BondArgs a)
     for (int s = 0; s < BPL; ++s) {
         S[s] = load_slot(a, F, F.t + s * LPF);
         alive[s] = S[s].act && a.broken[S[s].b] == 0;
-        g[s] = geometry64(a, S[s]);
+        g[s] = geometry64(a, S[s], a.A);
     }
     double kinv[BPL][3];
     bool sing[BPL];
@@ -582,9 +588,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 //    component (lane >> 4) of D_kb.  Families are taken one after the other and
 //    the results go back to the bond-per-lane layout through LDS.
 template <typename R, int MF> struct Contraction {
-    static constexpr bool MF32 = MF == 1 && sizeof(R) == 4;
+    static constexpr bool MF32 = (MF == 1 || MF == 4) && sizeof(R) == 4;
     static constexpr bool MF64 = MF == 1 && sizeof(R) == 8;
     static constexpr bool MW = MF == 2;
+    static constexpr bool RF = MF == 3 || MF == 4;   // rank-form weights (below)
 };
 
 // ---------------------------------------------------------------------------
@@ -616,14 +623,22 @@ template <typename R, int LPF, int BPL> struct MWL {
     static constexpr int FS = LPF * BPL;
     static constexpr int FPW = 64 / LPF;
     static constexpr int NB = FS / 16;
-    // slots per component row: an odd multiple of 16, so that the two halves
-    // of a wave (components c and c + 1) read distinct banks
+    // Row strides chosen against LDS bank conflicts (64 banks of 4 bytes, a
+    // wave served in 32-lane halves).  U / L rows (16 consecutive slots per
+    // lane group, rows hi = 0..3 a half-wave apart): an odd multiple of 16
+    // slots, so the two rows of a half land in distinct banks.  P rows (read
+    // with the component as the row index): in FP64 a stride of FSP + 8
+    // doubles puts the four rows a quarter of the banks apart.  Families are
+    // skewed by 8 doubles so the record writes of a wave's families spread.
     static constexpr int FSP = FS % 32 == 0 ? FS + 16 : FS;
+    static constexpr int PSP = sizeof(R) == 8 ? FSP + 8 : FSP;
     // per family 10 rows: U (3), L (3), P / Q (4); the fourth component of
     // the rank-3 Gram products is a zero the lanes supply themselves
     static constexpr int ROWS = 10, PROW = 6;
-    static constexpr int DOUBLES = (int)((FPW * ROWS * FSP * sizeof(R) + 7) / 8);
-    static __device__ __forceinline__ int at(int f, int c, int slot) { return (f * ROWS + c) * FSP + slot; }
+    static constexpr int FAM = 6 * FSP + 4 * PSP + (sizeof(R) == 8 ? 8 : 0);
+    static constexpr int DOUBLES = (int)((FPW * FAM * sizeof(R) + 7) / 8);
+    static __device__ __forceinline__ constexpr int row(int c) { return c < PROW ? c * FSP : PROW * FSP + (c - PROW) * PSP; }
+    static __device__ __forceinline__ int at(int f, int c, int slot) { return f * FAM + row(c) + slot; }
 };
 
 template <typename R, int N2_IS_10>
@@ -690,6 +705,7 @@ __device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& ho
 {
     typedef MWL<R, LPF, BPL> M;
     constexpr int FSP = M::FSP;
+    constexpr int PSP = M::PSP;
     constexpr int NB = M::NB;
     constexpr int PR = M::PROW;
     // families taken together (their chains interleave)
@@ -742,8 +758,8 @@ __device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& ho
 #pragma unroll
                 for (int e = 0; e < FG; ++e) {
                     R pa;
-                    if constexpr (sizeof(R) == 4) pa = base[e][(PR + lc) * FSP + 16 * s2 + 4 * hi + r];
-                    else pa = base[e][(PR + lc) * FSP + 16 * s2 + 4 * r + hi];
+                    if constexpr (sizeof(R) == 4) pa = base[e][PR * FSP + lc * PSP + 16 * s2 + 4 * hi + r];
+                    else pa = base[e][PR * FSP + lc * PSP + 16 * s2 + 4 * r + hi];
 #pragma unroll
                     for (int s1 = 0; s1 < NB; ++s1) {
                         if constexpr (sizeof(R) == 4)
@@ -776,16 +792,44 @@ __device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& ho
                     // lane l < 16 holds the four components of slot 16 s1 + l
                     if (lane < 16) {
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) base[e][(PR + c) * FSP + 16 * s1 + lane] = acc[e][s1][c];
+                        for (int c = 0; c < 4; ++c) base[e][PR * FSP + c * PSP + 16 * s1 + lane] = acc[e][s1][c];
                     }
                 } else {
                     // lane l holds component l / 16 of slot 16 s1 + l % 16
-                    base[e][(PR + hi) * FSP + 16 * s1 + lo] = acc[e][s1];
+                    base[e][PR * FSP + hi * PSP + 16 * s1 + lo] = acc[e][s1];
                 }
             }
         if (f0 == 0) hook_a();
     }
     hook_b();
+}
+
+// Rank-form weights (MF = 3: VALU contractions, MF = 4: FP32 matrix-core
+// contractions; tiles of 16 lanes per family).  The same identity as the
+// matrix-core weights, evaluated on the VALU: with the family's records in
+// length-rank order and lane t taking the bonds of ranks t, t + 16, t + 32 in
+// the contractions, record l lies above all of slot s's ranks when l / 16 > s
+// and below them when l / 16 < s, so
+//   w = (q_l (U_k0 + U_k1 e_xl + U_k2 e_yl))^n2 / 2^n2    (l / 16 > s)
+//   w = (p_l (L_k0 + L_k1 e_xl + L_k2 e_yl))^n2 / 2^n2    (l / 16 < s)
+// three VALU for the base and four for the tenth power, against ten for
+// exp(-A |l_k - l_l|) h^10 with its min; in the diagonal block (l / 16 = s)
+// both bases are formed and the smaller is the one that applies.  p, q here
+// are exp(+-A l / n2), and 1/2^n2 is folded into the P and Q records.
+template <typename R, int N2_IS_10, int FORM>
+__device__ __forceinline__ R weight_rf(const R* Uk, const R* Lk, R exl, R eyl, R pl, R ql, R n2)
+{
+    R g;
+    if constexpr (FORM > 0) {
+        g = (Uk[0] + Uk[1] * exl + Uk[2] * eyl) * ql;
+    } else if constexpr (FORM < 0) {
+        g = (Lk[0] + Lk[1] * exl + Lk[2] * eyl) * pl;
+    } else {
+        const R gu = (Uk[0] + Uk[1] * exl + Uk[2] * eyl) * ql;
+        const R gl = (Lk[0] + Lk[1] * exl + Lk[2] * eyl) * pl;
+        g = gu < gl ? gu : gl;
+    }
+    return mw_pow<N2_IS_10>(g, n2);
 }
 
 // LDS per wave of a bond-kernel variant, in doubles.
@@ -813,8 +857,10 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     constexpr bool MF32 = Contraction<R, MF>::MF32;
     constexpr bool MF64 = Contraction<R, MF>::MF64;
     constexpr bool MW = Contraction<R, MF>::MW;
+    constexpr bool RF = Contraction<R, MF>::RF;
     static_assert(!(MF64 || MW) || (Tile<LPF, BPL>::FS % 16 == 0),
                   "matrix-core contractions: families of a multiple of 16 slots");
+    static_assert(!RF || LPF == 16, "rank-form weights: 16 lanes per family");
     typedef typename Vec2<R>::T R2;
     constexpr int FS = Tile<LPF, BPL>::FS;
     constexpr int FPW = Tile<LPF, BPL>::FPW;
@@ -843,7 +889,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     if (nnew != 0 || a.force_k) {
         Geo<double> g64[BPL];
 #pragma unroll
-        for (int s = 0; s < BPL; ++s) g64[s] = geometry64(a, S[s]);
+        for (int s = 0; s < BPL; ++s) g64[s] = geometry64(a, S[s], a.A);
         bool sing[BPL];
         assemble_kinv<LPF, BPL, N2_IS_10>(rec64, F, S, alive, g64, a.n2, a.ktol, kinv, sing);
         if constexpr (!FIX) {
@@ -891,42 +937,47 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     // their own position, past the family's ranks), and the 2^-n2 of (g/2)^n2
     int mws[BPL];
     const R mw_scale = N2_IS_10 ? R(1.0 / 1024.0) : R(exp2(-a.n2));
+    if constexpr (MW || RF) {
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) mws[s] = S[s].act ? int(L.rk[s]) : S[s].k;
+    }
     if constexpr (MW) {
         typedef MWL<R, LPF, BPL> M;
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
-            mws[s] = S[s].act ? int(L.rk[s]) : S[s].k;
             R U[3], Lq[3];
             mw_geometry<R, N2_IS_10>(a, S[s], U, Lq);
             R* r0 = rec + M::at(F.slot, 0, mws[s]);
             r0[0] = U[0];
-            r0[M::FSP] = U[1];
-            r0[2 * M::FSP] = U[2];
-            r0[3 * M::FSP] = Lq[0];
-            r0[4 * M::FSP] = Lq[1];
-            r0[5 * M::FSP] = Lq[2];
+            r0[M::row(1)] = U[1];
+            r0[M::row(2)] = U[2];
+            r0[M::row(3)] = Lq[0];
+            r0[M::row(4)] = Lq[1];
+            r0[M::row(5)] = Lq[2];
             const R x0r = R(S[s].x0), x1r = R(S[s].x1);
             const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) * mw_scale : R(0);
             const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) * mw_scale : R(0);
-            r0[6 * M::FSP] = Vd0 * x0r;
-            r0[7 * M::FSP] = Vd0 * x1r;
-            r0[8 * M::FSP] = Vd1 * x0r;
-            r0[9 * M::FSP] = Vd1 * x1r;
+            r0[M::row(6)] = Vd0 * x0r;
+            r0[M::row(7)] = Vd0 * x1r;
+            r0[M::row(8)] = Vd1 * x0r;
+            r0[M::row(9)] = Vd1 * x1r;
         }
     }
     if constexpr (!MW) {
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
-        g[s] = GeoOf<R>::get(a, S[s]);
-        const int k = S[s].k;
+        // rank-form records carry exp(+-A l / n2) and the P scaled by 2^-n2
+        g[s] = GeoOf<R>::get(a, S[s], RF ? (N2_IS_10 ? a.A * 0.1 : a.A / a.n2) : a.A);
+        const int k = RF ? mws[s] : S[s].k;
         rec_put4<FPW>(rec, k, F.slot, 0, g[s].ex, g[s].ey, g[s].pe, g[s].qe);
         if constexpr (COMPACT) {
             const double Vl = alive[s] ? S[s].Vj * double(g[s].len) : 0.0;
             rec_put<FPW>(rec, k, F.slot, 2, R(Vl * S[s].d0), R(Vl * S[s].d1));
         } else {
             const R x0r = R(S[s].x0), x1r = R(S[s].x1);
-            const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) : R(0);
-            const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) : R(0);
+            const R sc = RF ? mw_scale : R(1);
+            const R Vd0 = alive[s] ? R(S[s].Vj * S[s].d0) * sc : R(0);
+            const R Vd1 = alive[s] ? R(S[s].Vj * S[s].d1) * sc : R(0);
             rec_put4<FPW>(rec, k, F.slot, 1, Vd0 * x0r, Vd0 * x1r, Vd1 * x0r, Vd1 * x1r);
         }
     }
@@ -935,6 +986,36 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     // run to the wave's largest family rounded up to 4 without guards
     const int mmax = wave_max_family<LPF>(F.M);
     wave_sync();
+    // rank form: lane t works on the rank slots t + 16 s in the contractions;
+    // its k-side factors U = p (1, e), L = q (1, e) of those slots
+    R Uk[RF ? BPL : 1][3], Lk[RF ? BPL : 1][3];
+    if constexpr (RF) {
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) {
+            const int kr = F.t + 16 * s;
+            const R ex = rec[rec_at<R, FPW>(kr, F.slot, 0)];
+            const R ey = rec[rec_at<R, FPW>(kr, F.slot, 1)];
+            const R pe = rec[rec_at<R, FPW>(kr, F.slot, 2)];
+            const R qe = rec[rec_at<R, FPW>(kr, F.slot, 3)];
+            Uk[s][0] = pe;
+            Uk[s][1] = pe * ex;
+            Uk[s][2] = pe * ey;
+            Lk[s][0] = qe;
+            Lk[s][1] = qe * ex;
+            Lk[s][2] = qe * ey;
+        }
+    }
+    // the weight of slot s's bond against record l
+    auto wt = [&](int s, int l, R ex, R ey, R pe, R qe) -> R {
+        if constexpr (RF) {
+            const int bl = l >> 4;
+            if (bl > s) return weight_rf<R, N2_IS_10, 1>(Uk[s], Lk[s], ex, ey, pe, qe, R(a.n2));
+            if (bl < s) return weight_rf<R, N2_IS_10, -1>(Uk[s], Lk[s], ex, ey, pe, qe, R(a.n2));
+            return weight_rf<R, N2_IS_10, 0>(Uk[s], Lk[s], ex, ey, pe, qe, R(a.n2));
+        } else {
+            return weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, R(a.n2));
+        }
+    };
 
     // ---- contraction 1: D_ik = sum_l w(k,l) V_l du_l (x) xi_l ----
     const R n2r = R(a.n2);
@@ -1053,7 +1134,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                     const R Pa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
-                        const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        const R om = wt(s, l, ex, ey, pe, qe);
                         if (s < NC) w[s][l] = om;
                         Dv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Pa, om, Dv[s], 0, 0, 0);
                     }
@@ -1062,7 +1143,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                     const R a1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
-                        const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        const R om = wt(s, l, ex, ey, pe, qe);
                         if (s < NC) w[s][l] = om;
                         const R t0 = om * a0, t1 = om * a1;
                         D[s][0] += t0 * ex;
@@ -1077,7 +1158,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                     const R P3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
-                        const R om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                        const R om = wt(s, l, ex, ey, pe, qe);
                         if (s < NC) w[s][l] = om;
                         D[s][0] += om * P0;
                         D[s][1] += om * P1;
@@ -1105,6 +1186,17 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
             for (int c = 0; c < 4; ++c) D[s][c] = Dv[s][c];
     }
+    }
+    if constexpr (RF) {
+        // D of the rank slots back to the lanes' own bonds, through the P rows
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) rec_put4<FPW>(rec, F.t + 16 * s, F.slot, 1, D[s][0], D[s][1], D[s][2], D[s][3]);
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(mws[s], F.slot, 4 + c)];
     }
 
     // ---- F - I = D K^-1, plane-stress sigma, Q_k = V_k sigma K^-1 ----
@@ -1159,10 +1251,13 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         if constexpr (MW) {
-            constexpr int FSPm = MWL<R, LPF, BPL>::FSP;
+            constexpr int PSPm = MWL<R, LPF, BPL>::PSP;
             R* r8 = rec + MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW, mws[s]);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) r8[c * FSPm] = Q[s][c] * mw_scale;
+            for (int c = 0; c < 4; ++c) r8[c * PSPm] = Q[s][c] * mw_scale;
+        } else if constexpr (RF) {
+            rec_put4<FPW>(rec, mws[s], F.slot, 1, Q[s][0] * mw_scale, Q[s][1] * mw_scale, Q[s][2] * mw_scale,
+                          Q[s][3] * mw_scale);
         } else {
             const int k = S[s].k;
             rec_put4<FPW>(rec, k, F.slot, 1, Q[s][0], Q[s][1], Q[s][2], Q[s][3]);
@@ -1256,7 +1351,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                     for (int s = 0; s < BPL; ++s) {
                         R om;
                         if (s >= NC)
-                            om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                            om = wt(s, l, ex, ey, pe, qe);
                         else
                             om = w[s][l];
                         Sv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Qa, om, Sv[s], 0, 0, 0);
@@ -1277,7 +1372,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                     for (int s = 0; s < BPL; ++s) {
                         R om;
                         if (s >= NC)
-                            om = weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, n2r);
+                            om = wt(s, l, ex, ey, pe, qe);
                         else
                             om = w[s][l];
                         Sm[s][0] += om * Q0;
@@ -1296,6 +1391,18 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
             for (int c = 0; c < 4; ++c) Sm[s][c] = Sv[s][c];
     }
+    }
+    if constexpr (RF) {
+        // S of the rank slots back to the lanes' own bonds
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+            rec_put4<FPW>(rec, F.t + 16 * s, F.slot, 1, Sm[s][0], Sm[s][1], Sm[s][2], Sm[s][3]);
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(mws[s], F.slot, 4 + c)];
     }
     // V_H and the own-bond force sum accumulate in the bond arithmetic's
     // precision (FP32 in mixed precision, like the T they sum; measured -2 %)
@@ -1358,7 +1465,7 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
     FamLoad<R, BPL> L;
     stage_rows<LPF>(a, L, lane, blk * wpb + wib);
-    stage_cols<R, LPF, BPL, DMG, 0, MF == 2>(a, L);
+    stage_cols<R, LPF, BPL, DMG, 0, MF >= 2>(a, L);
     stage_bonds<R, LPF, BPL>(a, L);
     bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * WaveLds<R, LPF, BPL, MF>::DOUBLES, lane,
                                               NoHook(), NoHook());
@@ -1387,7 +1494,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
     if (g >= g1) return;
     FamLoad<R, BPL> cur, nxt;
     stage_rows<LPF>(a, cur, lane, g);
-    stage_cols<R, LPF, BPL, DMG, 0, MF == 2>(a, cur);
+    stage_cols<R, LPF, BPL, DMG, 0, MF >= 2>(a, cur);
     stage_bonds<R, LPF, BPL>(a, cur);
     for (;;) {
         const int gn = g + wpb;
@@ -1399,7 +1506,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
         stage_rows<LPF>(a, nxt, lane, gp);
         bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(
             a, cur, rec64, lane,
-            [&]() { stage_cols<R, LPF, BPL, DMG, 0, MF == 2>(a, nxt); },
+            [&]() { stage_cols<R, LPF, BPL, DMG, 0, MF >= 2>(a, nxt); },
             [&]() { stage_bonds<R, LPF, BPL>(a, nxt); });
         if (!more) break;
         cur = nxt;

@@ -1,7 +1,8 @@
 // pd_bond_f64.hip -- FP64 bond-kernel variants: tiles (32,1), (64,1), (16,2),
-// (16,3) with VALU or matrix-core contractions, and (32,1), (64,1), (16,3)
-// with matrix-core weights, each with and without the damage update; the FP64
-// fix-up of the guarded families of a mixed-precision step.
+// (16,3) with VALU or matrix-core contractions, (32,1), (64,1), (16,3) with
+// matrix-core weights, (16,2), (16,3) with rank-form weights, each with and
+// without the damage update; the FP64 fix-up of the guarded families of a
+// mixed-precision step.
 #include "pd_bond_launch.h"
 
 namespace pd {
@@ -9,7 +10,15 @@ namespace pd {
 hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s)
 {
-    if (mf == 2) {
+    if (mf == 3) {
+        if (dmg) {
+            if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2, 3, 1>(a, n2_is_10, wpb, pgrid, s);
+            if (bpl == 3 && lpf == 16) return launch_bond_n2<double, 16, 3, 3, 1>(a, n2_is_10, wpb, pgrid, s);
+        } else {
+            if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2, 3>(a, n2_is_10, wpb, pgrid, s);
+            if (bpl == 3 && lpf == 16) return launch_bond_n2<double, 16, 3, 3>(a, n2_is_10, wpb, pgrid, s);
+        }
+    } else if (mf == 2) {
         if (dmg) {
             if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1, 2, 1>(a, n2_is_10, wpb, pgrid, s);
             if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1, 2, 1>(a, n2_is_10, wpb, pgrid, s);

@@ -574,9 +574,9 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
             lpf = c->maxM <= 32 ? 32 : 64;
             bpl = 1;
         }
-        if (mf == 2 && !(c->p.n2 > 0))
-            return fail(c, PD_ERR_INVALID, "matrix-core weights need an angular exponent n2 > 0");
-        if (value < 0 || mf > 2 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
+        if (mf >= 2 && !(c->p.n2 > 0))
+            return fail(c, PD_ERR_INVALID, "matrix-core and rank-form weights need an angular exponent n2 > 0");
+        if (value < 0 || mf > 4 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
             lpf * bpl < c->maxM)
             return fail(c, PD_ERR_INVALID, "tile %ld (lanes %d x bonds %d, matrix cores %d) unsupported for "
                         "this precision or smaller than the largest family (%d)", value, lpf, bpl, mf, c->maxM);
@@ -1169,10 +1169,10 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     std::vector<Var> vars;
     const int mixed = c->p.precision == PD_MIXED;
     const int tiles[5][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}, {16, 3}};
-    for (int mf : {0, 1, 2})
+    for (int mf : {0, 1, 2, 3, 4})
         for (const auto& tl : tiles) {
             if (!tile_supported(mixed, tl[0], tl[1], mf) || tl[0] * tl[1] < c->maxM) continue;
-            if (mf == 2 && !(c->p.n2 > 0)) continue;
+            if (mf >= 2 && !(c->p.n2 > 0)) continue;
             for (int w : {1, 2, 4, 8})
                 for (int x : {1, 8})
                     for (int pz : {0, 1, 2, 4})

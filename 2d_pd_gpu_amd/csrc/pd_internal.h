@@ -84,7 +84,9 @@ struct PosArgs {
 // lpf * bpl bonds.
 // mf = 1: family contractions on the matrix cores (every tile except FP64
 // (32,2), which does not exist).  mf = 2 (FP64 only): matrix-core weights and
-// contractions (tiles (32,1), (64,1), (16,3); n2 > 0).
+// contractions (tiles (32,1), (64,1), (16,3); n2 > 0).  mf = 3: rank-form
+// weights, VALU contractions; mf = 4 (mixed only): rank-form weights with
+// matrix-core contractions (tiles (16,2), (16,3); n2 > 0).
 bool tile_supported(int mixed, int lpf, int bpl, int mf);
 // pgrid > 0: persistent launch with pgrid workgroups (each walks a contiguous
 // range of family groups, prefetching the next group's loads).
