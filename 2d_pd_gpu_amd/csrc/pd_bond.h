@@ -589,9 +589,11 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 //    the results go back to the bond-per-lane layout through LDS.
 template <typename R, int MF> struct Contraction {
     static constexpr bool MF32 = (MF == 1 || MF == 4) && sizeof(R) == 4;
-    static constexpr bool MF64 = MF == 1 && sizeof(R) == 8;
+    static constexpr bool MF64 = (MF == 1 || MF == 5) && sizeof(R) == 8;
     static constexpr bool MW = MF == 2;
-    static constexpr bool RF = MF == 3 || MF == 4;   // rank-form weights (below)
+    // rank-form weights (below): MF = 3 VALU contractions, MF = 4 FP32 and
+    // MF = 5 FP64 matrix-core contractions
+    static constexpr bool RF = MF == 3 || MF == 4 || MF == 5;
 };
 
 // ---------------------------------------------------------------------------
@@ -748,7 +750,7 @@ __device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& ho
                     } else {
                         const G4 gu = mw_gram(Lq[e][s2], U[e][s1]), gl = mw_gram(U[e][s2], Lq[e][s1]);
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) g[r] = gu[r] < gl[r] ? gu[r] : gl[r];
+                        for (int r = 0; r < 4; ++r) g[r] = rmin(gu[r], gl[r]);
                     }
 #pragma unroll
                     for (int r = 0; r < 4; ++r) w[e][s1][r] = mw_pow<N2_IS_10>(R(g[r]), n2);
@@ -805,32 +807,114 @@ __device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& ho
 }
 
 // Rank-form weights (MF = 3: VALU contractions, MF = 4: FP32 matrix-core
-// contractions; tiles of 16 lanes per family).  The same identity as the
+// contractions, both on tiles of 16 lanes per family; MF = 5: FP64 matrix-core
+// contractions, any tile of a multiple of 16 slots).  The same identity as the
 // matrix-core weights, evaluated on the VALU: with the family's records in
 // length-rank order and lane t taking the bonds of ranks t, t + 16, t + 32 in
-// the contractions, record l lies above all of slot s's ranks when l / 16 > s
-// and below them when l / 16 < s, so
+// the contractions (MF = 5: bonds of ranks 16 s + (lane & 15) against the
+// neighbour ranks 4 q + (lane >> 4), so l / 16 = q / 4 is wave-uniform),
+// record l lies above all of slot s's ranks when l / 16 > s and below them
+// when l / 16 < s, so
 //   w = (q_l (U_k0 + U_k1 e_xl + U_k2 e_yl))^n2 / 2^n2    (l / 16 > s)
 //   w = (p_l (L_k0 + L_k1 e_xl + L_k2 e_yl))^n2 / 2^n2    (l / 16 < s)
 // three VALU for the base and four for the tenth power, against ten for
-// exp(-A |l_k - l_l|) h^10 with its min; in the diagonal block (l / 16 = s)
-// both bases are formed and the smaller is the one that applies.  p, q here
-// are exp(+-A l / n2), and 1/2^n2 is folded into the P and Q records.
+// exp(-A |l_k - l_l|) h^10 with its min.  In the diagonal block (l / 16 = s)
+// the form follows from the pair: L_k = r_k U_k with r_k = q_k / p_k = q_k^2,
+// so the L form is the U form's base times r_k p_l instead of q_l, and the
+// smaller of the two applies: g = (U_k . (1, e_l)) min(q_l, r_k p_l), five
+// VALU for the base.  p, q here are exp(+-A l / n2), and 1/2^n2 is folded into
+// the P and Q records.  k holds U_k (0..2), L_k (3..5) and r_k (6).
 template <typename R, int N2_IS_10, int FORM>
-__device__ __forceinline__ R weight_rf(const R* Uk, const R* Lk, R exl, R eyl, R pl, R ql, R n2)
+__device__ __forceinline__ R weight_rf(const R* k, R exl, R eyl, R pl, R ql, R n2)
 {
     R g;
     if constexpr (FORM > 0) {
-        g = (Uk[0] + Uk[1] * exl + Uk[2] * eyl) * ql;
+        g = (k[0] + k[1] * exl + k[2] * eyl) * ql;
     } else if constexpr (FORM < 0) {
-        g = (Lk[0] + Lk[1] * exl + Lk[2] * eyl) * pl;
+        g = (k[3] + k[4] * exl + k[5] * eyl) * pl;
     } else {
-        const R gu = (Uk[0] + Uk[1] * exl + Uk[2] * eyl) * ql;
-        const R gl = (Lk[0] + Lk[1] * exl + Lk[2] * eyl) * pl;
-        g = gu < gl ? gu : gl;
+        g = (k[0] + k[1] * exl + k[2] * eyl) * rmin(ql, k[6] * pl);
     }
     return mw_pow<N2_IS_10>(g, n2);
 }
+
+// The k-side factors of the rank form from a bond's record values.
+template <typename R>
+__device__ __forceinline__ void rf_kside(R ex, R ey, R pe, R qe, R* k)
+{
+    k[0] = pe;
+    k[1] = pe * ex;
+    k[2] = pe * ey;
+    k[3] = qe;
+    k[4] = qe * ex;
+    k[5] = qe * ey;
+    k[6] = qe * qe;
+}
+
+// k-side operands of the FP64 matrix-core contractions for one family: the
+// bonds of record rows 16 s + lk (s < NS).  Classic weights keep the half unit
+// vector and the radial factors; the rank form keeps U = p (1, e) and
+// L = q (1, e), and picks its form from the neighbour block q / 4 (neighbour
+// rows 4 q .. 4 q + 3) against s, both known at compile time once the loops
+// are unrolled.
+template <int NS, bool RFK> struct MfK {
+    double v[NS][RFK ? 7 : 4];
+
+    template <typename R, int FPW>
+    __device__ __forceinline__ void load(const R* rec, int f, int lk)
+    {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int kb = 16 * s + lk;
+            const double ex = rec[rec_at<R, FPW>(kb, f, 0)];
+            const double ey = rec[rec_at<R, FPW>(kb, f, 1)];
+            const double pe = rec[rec_at<R, FPW>(kb, f, 2)];
+            const double qe = rec[rec_at<R, FPW>(kb, f, 3)];
+            if constexpr (RFK) {
+                rf_kside(ex, ey, pe, qe, v[s]);
+            } else {
+                v[s][0] = 0.5 * ex;
+                v[s][1] = 0.5 * ey;
+                v[s][2] = pe;
+                v[s][3] = qe;
+            }
+        }
+    }
+
+    template <int N2_IS_10>
+    __device__ __forceinline__ double w(int s, int q, double ex, double ey, double pe, double qe, double n2) const
+    {
+        if constexpr (RFK) {
+            const int bl = q >> 2;
+            if (bl > s) return weight_rf<double, N2_IS_10, 1>(v[s], ex, ey, pe, qe, n2);
+            if (bl < s) return weight_rf<double, N2_IS_10, -1>(v[s], ex, ey, pe, qe, n2);
+            return weight_rf<double, N2_IS_10, 0>(v[s], ex, ey, pe, qe, n2);
+        } else {
+            return weight<double, N2_IS_10>(v[s][0], v[s][1], v[s][2], v[s][3], ex, ey, pe, qe, n2);
+        }
+    }
+};
+
+// Neighbour-side operands of one step of the FP64 matrix-core contractions:
+// the geometry of record row lb (read when the weights are evaluated) and
+// component lc of its P / Q values.
+struct MfL {
+    double ex, ey, pe, qe, y;
+
+    template <typename R, int FPW, bool GEO = true>
+    __device__ __forceinline__ void load(const R* rec, int f, int lb, int lc)
+    {
+        if constexpr (GEO) {
+            ex = rec[rec_at<R, FPW>(lb, f, 0)];
+            ey = rec[rec_at<R, FPW>(lb, f, 1)];
+            pe = rec[rec_at<R, FPW>(lb, f, 2)];
+            qe = rec[rec_at<R, FPW>(lb, f, 3)];
+        } else {
+            ex = ey = pe = qe = 0.0;
+        }
+        y = rec[rec_at<R, FPW>(lb, f, 4 + lc)];
+    }
+};
 
 // LDS per wave of a bond-kernel variant, in doubles.
 template <typename R, int LPF, int BPL, int MF> struct WaveLds {
@@ -860,7 +944,9 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     constexpr bool RF = Contraction<R, MF>::RF;
     static_assert(!(MF64 || MW) || (Tile<LPF, BPL>::FS % 16 == 0),
                   "matrix-core contractions: families of a multiple of 16 slots");
-    static_assert(!RF || LPF == 16, "rank-form weights: 16 lanes per family");
+    static_assert(!RF || MF64 || LPF == 16, "rank-form VALU / FP32 contractions: 16 lanes per family");
+    // rank-form weights with the lane-per-rank-slot contractions (MF = 3, 4)
+    constexpr bool RFL = RF && !MF64;
     typedef typename Vec2<R>::T R2;
     constexpr int FS = Tile<LPF, BPL>::FS;
     constexpr int FPW = Tile<LPF, BPL>::FPW;
@@ -987,31 +1073,23 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     const int mmax = wave_max_family<LPF>(F.M);
     wave_sync();
     // rank form: lane t works on the rank slots t + 16 s in the contractions;
-    // its k-side factors U = p (1, e), L = q (1, e) of those slots
-    R Uk[RF ? BPL : 1][3], Lk[RF ? BPL : 1][3];
-    if constexpr (RF) {
+    // its k-side factors U = p (1, e), L = q (1, e), r = q / p of those slots
+    R Kk[RFL ? BPL : 1][7];
+    if constexpr (RFL) {
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
             const int kr = F.t + 16 * s;
-            const R ex = rec[rec_at<R, FPW>(kr, F.slot, 0)];
-            const R ey = rec[rec_at<R, FPW>(kr, F.slot, 1)];
-            const R pe = rec[rec_at<R, FPW>(kr, F.slot, 2)];
-            const R qe = rec[rec_at<R, FPW>(kr, F.slot, 3)];
-            Uk[s][0] = pe;
-            Uk[s][1] = pe * ex;
-            Uk[s][2] = pe * ey;
-            Lk[s][0] = qe;
-            Lk[s][1] = qe * ex;
-            Lk[s][2] = qe * ey;
+            rf_kside(rec[rec_at<R, FPW>(kr, F.slot, 0)], rec[rec_at<R, FPW>(kr, F.slot, 1)],
+                     rec[rec_at<R, FPW>(kr, F.slot, 2)], rec[rec_at<R, FPW>(kr, F.slot, 3)], Kk[s]);
         }
     }
     // the weight of slot s's bond against record l
     auto wt = [&](int s, int l, R ex, R ey, R pe, R qe) -> R {
-        if constexpr (RF) {
+        if constexpr (RFL) {
             const int bl = l >> 4;
-            if (bl > s) return weight_rf<R, N2_IS_10, 1>(Uk[s], Lk[s], ex, ey, pe, qe, R(a.n2));
-            if (bl < s) return weight_rf<R, N2_IS_10, -1>(Uk[s], Lk[s], ex, ey, pe, qe, R(a.n2));
-            return weight_rf<R, N2_IS_10, 0>(Uk[s], Lk[s], ex, ey, pe, qe, R(a.n2));
+            if (bl > s) return weight_rf<R, N2_IS_10, 1>(Kk[s], ex, ey, pe, qe, R(a.n2));
+            if (bl < s) return weight_rf<R, N2_IS_10, -1>(Kk[s], ex, ey, pe, qe, R(a.n2));
+            return weight_rf<R, N2_IS_10, 0>(Kk[s], ex, ey, pe, qe, R(a.n2));
         } else {
             return weight<R, N2_IS_10>(g[s].hx, g[s].hy, g[s].pe, g[s].qe, ex, ey, pe, qe, R(a.n2));
         }
@@ -1070,35 +1148,26 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int f = 0; f < WF; ++f) {
             Mf[f] = __builtin_amdgcn_readfirstlane(__shfl(F.M, f * LPF, 64));
-            double hk[NS], yk[NS], pk[NS], qk[NS];
+            MfK<NS, RF> ks;
+            ks.template load<R, FPW>(rec, f, lk);
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const int kb = 16 * s + lk;
-                hk[s] = 0.5 * rec[rec_at<R, FPW>(kb, f, 0)];
-                yk[s] = 0.5 * rec[rec_at<R, FPW>(kb, f, 1)];
-                pk[s] = rec[rec_at<R, FPW>(kb, f, 2)];
-                qk[s] = rec[rec_at<R, FPW>(kb, f, 3)];
-                Dacc[f][s] = 0.0;
-            }
+            for (int s = 0; s < NS; ++s) Dacc[f][s] = 0.0;
+            // neighbour rows 4 q + lq, loaded one step ahead of their use
+            MfL nb[2];
+            nb[0].template load<R, FPW>(rec, f, lq, lc);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
+                if (q + 1 < NQ) nb[(q + 1) & 1].template load<R, FPW>(rec, f, 4 * (q + 1) + lq, lc);
                 if (4 * q < Mf[f]) {
-                    const int lb = 4 * q + lq;
-                    const double ex = rec[rec_at<R, FPW>(lb, f, 0)];
-                    const double ey = rec[rec_at<R, FPW>(lb, f, 1)];
-                    const double pe = rec[rec_at<R, FPW>(lb, f, 2)];
-                    const double qe = rec[rec_at<R, FPW>(lb, f, 3)];
-                    const double Pa = rec[rec_at<R, FPW>(lb, f, 4 + lc)];
+                    const MfL& r = nb[q & 1];
+                    // every bond set, also those past the family's end (their
+                    // records are zero, so are their weights): no branch
+                    // between the independent weight chains
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
-                        if (16 * s < Mf[f]) {
-                            const double om = weight<double, N2_IS_10>(hk[s], yk[s], pk[s], qk[s], ex, ey,
-                                                                       pe, qe, a.n2);
-                            if constexpr (!MFRW) wc[f][q][s] = om;
-                            Dacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(Pa, om, Dacc[f][s], 0, 0, 0);
-                        } else {
-                            if constexpr (!MFRW) wc[f][q][s] = 0.0;
-                        }
+                        const double om = ks.template w<N2_IS_10>(s, q, r.ex, r.ey, r.pe, r.qe, a.n2);
+                        if constexpr (!MFRW) wc[f][q][s] = om;
+                        Dacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(r.y, om, Dacc[f][s], 0, 0, 0);
                     }
                 } else if constexpr (!MFRW) {
 #pragma unroll
@@ -1119,7 +1188,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
+            for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(RF ? mws[s] : S[s].k, F.slot, 4 + c)];
     } else {
 #pragma unroll
     for (int l0 = 0; l0 < FS; l0 += GL) {
@@ -1187,7 +1256,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             for (int c = 0; c < 4; ++c) D[s][c] = Dv[s][c];
     }
     }
-    if constexpr (RF) {
+    if constexpr (RFL) {
         // D of the rank slots back to the lanes' own bonds, through the P rows
         wave_sync();
 #pragma unroll
@@ -1281,40 +1350,26 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int f = 0; f < WF; ++f) {
             // the records' geometry values (0..3) are still in place
-            double hk[NS], yk[NS], pk[NS], qk[NS];
+            MfK<NS, RF> ks;
+            if constexpr (MFRW) ks.template load<R, FPW>(rec, f, lk);
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                Sacc[f][s] = 0.0;
-                if constexpr (MFRW) {
-                    const int kb = 16 * s + lk;
-                    hk[s] = 0.5 * rec[rec_at<R, FPW>(kb, f, 0)];
-                    yk[s] = 0.5 * rec[rec_at<R, FPW>(kb, f, 1)];
-                    pk[s] = rec[rec_at<R, FPW>(kb, f, 2)];
-                    qk[s] = rec[rec_at<R, FPW>(kb, f, 3)];
-                }
-            }
+            for (int s = 0; s < NS; ++s) Sacc[f][s] = 0.0;
+            MfL nb[2];
+            nb[0].template load<R, FPW, MFRW>(rec, f, lq, lc);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
+                if (q + 1 < NQ) nb[(q + 1) & 1].template load<R, FPW, MFRW>(rec, f, 4 * (q + 1) + lq, lc);
                 if (4 * q < Mf[f]) {
-                    const int lb = 4 * q + lq;
-                    const double Qa = rec[rec_at<R, FPW>(lb, f, 4 + lc)];
-                    double ex = 0.0, ey = 0.0, pe = 0.0, qe = 0.0;
-                    if constexpr (MFRW) {
-                        ex = rec[rec_at<R, FPW>(lb, f, 0)];
-                        ey = rec[rec_at<R, FPW>(lb, f, 1)];
-                        pe = rec[rec_at<R, FPW>(lb, f, 2)];
-                        qe = rec[rec_at<R, FPW>(lb, f, 3)];
-                    }
+                    const MfL& r = nb[q & 1];
 #pragma unroll
-                    for (int s = 0; s < NS; ++s)
-                        if (16 * s < Mf[f]) {
-                            double om;
-                            if constexpr (MFRW)
-                                om = weight<double, N2_IS_10>(hk[s], yk[s], pk[s], qk[s], ex, ey, pe, qe, a.n2);
-                            else
-                                om = wc[f][q][s];
-                            Sacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(Qa, om, Sacc[f][s], 0, 0, 0);
-                        }
+                    for (int s = 0; s < NS; ++s) {
+                        double om;
+                        if constexpr (MFRW)
+                            om = ks.template w<N2_IS_10>(s, q, r.ex, r.ey, r.pe, r.qe, a.n2);
+                        else
+                            om = wc[f][q][s];
+                        Sacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(r.y, om, Sacc[f][s], 0, 0, 0);
+                    }
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -1328,7 +1383,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
+            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(RF ? mws[s] : S[s].k, F.slot, 4 + c)];
     } else {
     v4f Sv[BPL];
 #pragma unroll
@@ -1392,7 +1447,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             for (int c = 0; c < 4; ++c) Sm[s][c] = Sv[s][c];
     }
     }
-    if constexpr (RF) {
+    if constexpr (RFL) {
         // S of the rank slots back to the lanes' own bonds
         wave_sync();
 #pragma unroll

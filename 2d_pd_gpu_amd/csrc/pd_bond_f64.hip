@@ -1,8 +1,7 @@
-// pd_bond_f64.hip -- FP64 bond-kernel variants: tiles (32,1), (64,1), (16,2),
-// (16,3) with VALU or matrix-core contractions, (32,1), (64,1), (16,3) with
-// matrix-core weights, (16,2), (16,3) with rank-form weights, each with and
-// without the damage update; the FP64 fix-up of the guarded families of a
-// mixed-precision step.
+// pd_bond_f64.hip -- FP64 bond-kernel variants with records in bond order:
+// tiles (32,1), (64,1), (16,2), (16,3) with VALU or matrix-core contractions,
+// each with and without the damage update; the FP64 fix-up of the guarded
+// families of a mixed-precision step.
 #include "pd_bond_launch.h"
 
 namespace pd {
@@ -10,25 +9,9 @@ namespace pd {
 hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s)
 {
-    if (mf == 3) {
-        if (dmg) {
-            if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2, 3, 1>(a, n2_is_10, wpb, pgrid, s);
-            if (bpl == 3 && lpf == 16) return launch_bond_n2<double, 16, 3, 3, 1>(a, n2_is_10, wpb, pgrid, s);
-        } else {
-            if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2, 3>(a, n2_is_10, wpb, pgrid, s);
-            if (bpl == 3 && lpf == 16) return launch_bond_n2<double, 16, 3, 3>(a, n2_is_10, wpb, pgrid, s);
-        }
-    } else if (mf == 2) {
-        if (dmg) {
-            if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1, 2, 1>(a, n2_is_10, wpb, pgrid, s);
-            if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1, 2, 1>(a, n2_is_10, wpb, pgrid, s);
-            if (bpl == 3 && lpf == 16) return launch_bond_n2<double, 16, 3, 2, 1>(a, n2_is_10, wpb, pgrid, s);
-        } else {
-            if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1, 2>(a, n2_is_10, wpb, pgrid, s);
-            if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1, 2>(a, n2_is_10, wpb, pgrid, s);
-            if (bpl == 3 && lpf == 16) return launch_bond_n2<double, 16, 3, 2>(a, n2_is_10, wpb, pgrid, s);
-        }
-    } else if (dmg && mf) {
+    // the variants with records in length-rank order build in their own unit
+    if (mf >= 2) return launch_bond_f64_rank(a, lpf, bpl, mf, dmg, n2_is_10, wpb, pgrid, s);
+    if (dmg && mf) {
         if (bpl == 1 && lpf == 32) return launch_bond_n2<double, 32, 1, 1, 1>(a, n2_is_10, wpb, pgrid, s);
         if (bpl == 1 && lpf == 64) return launch_bond_n2<double, 64, 1, 1, 1>(a, n2_is_10, wpb, pgrid, s);
         if (bpl == 2 && lpf == 16) return launch_bond_n2<double, 16, 2, 1, 1>(a, n2_is_10, wpb, pgrid, s);

@@ -576,7 +576,7 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
         }
         if (mf >= 2 && !(c->p.n2 > 0))
             return fail(c, PD_ERR_INVALID, "matrix-core and rank-form weights need an angular exponent n2 > 0");
-        if (value < 0 || mf > 4 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
+        if (value < 0 || mf > 5 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
             lpf * bpl < c->maxM)
             return fail(c, PD_ERR_INVALID, "tile %ld (lanes %d x bonds %d, matrix cores %d) unsupported for "
                         "this precision or smaller than the largest family (%d)", value, lpf, bpl, mf, c->maxM);
@@ -1169,7 +1169,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     std::vector<Var> vars;
     const int mixed = c->p.precision == PD_MIXED;
     const int tiles[5][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}, {16, 3}};
-    for (int mf : {0, 1, 2, 3, 4})
+    for (int mf : {0, 1, 2, 3, 4, 5})
         for (const auto& tl : tiles) {
             if (!tile_supported(mixed, tl[0], tl[1], mf) || tl[0] * tl[1] < c->maxM) continue;
             if (mf >= 2 && !(c->p.n2 > 0)) continue;

@@ -95,6 +95,8 @@ hipError_t launch_bond(const BondArgs& a, int mixed, int lpf, int bpl, int mf, i
                        int wpb, int pgrid, hipStream_t s);
 hipError_t launch_bond_f64(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s);
+hipError_t launch_bond_f64_rank(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
+                                int pgrid, hipStream_t s);
 hipError_t launch_bond_f32(const BondArgs& a, int lpf, int bpl, int mf, int dmg, int n2_is_10, int wpb,
                            int pgrid, hipStream_t s);
 hipError_t launch_shape(const BondArgs& a, int lpf, int bpl, int n2_is_10, int wpb, hipStream_t s);

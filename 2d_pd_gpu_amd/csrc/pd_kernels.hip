@@ -234,6 +234,7 @@ bool tile_supported(int mixed, int lpf, int bpl, int mf)
     if (mf == 2) return !mixed && ((bpl == 1 && (lpf == 32 || lpf == 64)) || (lpf == 16 && bpl == 3));
     if (mf == 3) return lpf == 16 && (bpl == 2 || bpl == 3);
     if (mf == 4) return mixed && lpf == 16 && (bpl == 2 || bpl == 3);
+    if (mf == 5) return !mixed && ((bpl == 1 && (lpf == 32 || lpf == 64)) || (lpf == 16 && (bpl == 2 || bpl == 3)));
     if (mf && !mixed) return (bpl == 1 && (lpf == 32 || lpf == 64)) || (lpf == 16 && (bpl == 2 || bpl == 3));
     if (bpl == 1) return lpf == 32 || lpf == 64;
     if (bpl == 2) return lpf == 16 || (lpf == 32 && mixed);

@@ -73,7 +73,8 @@ extern "C" {
                                    23201, 26401, 21603 (PD_FP64, n2 > 0); rank-form
                                    weights 31602, 31603 (and 41602, 41603 with
                                    matrix-core contractions in mixed precision;
-                                   n2 > 0);
+                                   53201, 56401, 51602, 51603 with matrix-core
+                                   contractions in PD_FP64; n2 > 0);
                                    0 = 3201 or 6401 by the largest family            */
 #define PD_OPT_PERSISTENT    6  /* bond kernel as a persistent grid of value workgroups
                                    per CU that prefetch their next family group
