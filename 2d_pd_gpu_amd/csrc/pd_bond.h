@@ -98,7 +98,9 @@ __device__ __forceinline__ bool mazars_update(const BondArgs& a, double e11, dou
 }
 
 __device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
-__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+// FP32: the median of (a, b, -inf) is min(a, b) in one v_med3_f32 (fminf
+// would first quiet each operand loaded from memory)
+__device__ __forceinline__ float rmin(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -__builtin_inff()); }
 __device__ __forceinline__ double rmax(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double rpow(double a, double b) { return pow(a, b); }
@@ -498,7 +500,6 @@ template <typename R, int BPL> struct FamLoad {
     double Vj[BPL];
     double kap[BPL], dm[BPL];  // damage threshold and value (damage kernels)
     unsigned char ff;          // the family's guard flag (mixed precision)
-    unsigned char rk[BPL];     // rank of the bond's length in its family (MW contractions)
     static constexpr bool GUARD = sizeof(R) == 4;
 };
 
@@ -509,7 +510,7 @@ __device__ __forceinline__ void stage_rows(const BondArgs& a, FL& L, int lane, i
     if constexpr (FL::GUARD) L.ff = a.famflag[L.F.famok ? L.F.fam : 0];
 }
 
-template <typename R, int LPF, int BPL, int DMG = 0, int FIX = 0, int RK = 0>
+template <typename R, int LPF, int BPL, int DMG = 0, int FIX = 0>
 __device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<R, BPL>& L)
 {
     family_extent(L.F);
@@ -536,7 +537,6 @@ __device__ __forceinline__ void stage_cols(const BondArgs& a, FamLoad<R, BPL>& L
             L.kap[s] = a.kappa[b];
             L.dm[s] = a.dmg[b];
         }
-        if constexpr (RK) L.rk[s] = a.rank[b];
     }
 }
 
@@ -589,11 +589,9 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 //    the results go back to the bond-per-lane layout through LDS.
 template <typename R, int MF> struct Contraction {
     static constexpr bool MF32 = (MF == 1 || MF == 4) && sizeof(R) == 4;
-    static constexpr bool MF64 = (MF == 1 || MF == 5) && sizeof(R) == 8;
+    static constexpr bool MF64 = MF == 1 && sizeof(R) == 8;
     static constexpr bool MW = MF == 2;
-    // rank-form weights (below): MF = 3 VALU contractions, MF = 4 FP32 and
-    // MF = 5 FP64 matrix-core contractions
-    static constexpr bool RF = MF == 3 || MF == 4 || MF == 5;
+    static constexpr bool RF = MF == 3 || MF == 4;   // rank-form weights (below)
 };
 
 // ---------------------------------------------------------------------------
@@ -605,8 +603,8 @@ template <typename R, int MF> struct Contraction {
 //   g = U_k . L_l (l_l >= l_k)  or  L_k . U_l (l_l <= l_k),
 // where U = p (1, e_x, e_y) and L = q (1, e_x, e_y): the radial factor folds
 // into the base of the power, and which of the two forms applies follows from
-// the order of the lengths.  Each family's bonds are given their length rank
-// at set-up (rank[]), and their records are written to LDS in rank order, so in
+// the order of the lengths.  Each family is stored in ascending order of its
+// bonds' lengths, so its records are in rank order, and in
 // a 16 x 16 block of (bond, neighbour) ranks every pair has the same form,
 // except on the diagonal blocks, which take the smaller of the two (it is the
 // one that applies).  Per 16 x 16 block:
@@ -807,14 +805,16 @@ __device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& ho
 }
 
 // Rank-form weights (MF = 3: VALU contractions, MF = 4: FP32 matrix-core
-// contractions, both on tiles of 16 lanes per family; MF = 5: FP64 matrix-core
-// contractions, any tile of a multiple of 16 slots).  The same identity as the
-// matrix-core weights, evaluated on the VALU: with the family's records in
-// length-rank order and lane t taking the bonds of ranks t, t + 16, t + 32 in
-// the contractions (MF = 5: bonds of ranks 16 s + (lane & 15) against the
-// neighbour ranks 4 q + (lane >> 4), so l / 16 = q / 4 is wave-uniform),
-// record l lies above all of slot s's ranks when l / 16 > s and below them
-// when l / 16 < s, so
+// contractions; tiles of 16 lanes per family).  Each family is stored in
+// ascending order of its bonds' reference lengths (ties in the caller's
+// order), and lane t owns the bonds t, t + 16, t + 32: slot s holds bond
+// (rank) 16 s + t.  With p = exp(A l / n2), q = 1 / p, h = 1/2 + 1/2 e_k.e_l,
+// U = p (1, e), L = q (1, e),
+//   w(k,l) = exp(-A |l_k - l_l|) h^n2 = (g / 2)^n2,
+//   g = U_k . L_l (l_l >= l_k)  or  L_k . U_l (l_l <= l_k):
+// the radial factor folds into the base of the power, and which of the two
+// forms applies follows from the order of the lengths.  Record l lies above
+// all of slot s's bonds when l / 16 > s and below them when l / 16 < s, so
 //   w = (q_l (U_k0 + U_k1 e_xl + U_k2 e_yl))^n2 / 2^n2    (l / 16 > s)
 //   w = (p_l (L_k0 + L_k1 e_xl + L_k2 e_yl))^n2 / 2^n2    (l / 16 < s)
 // three VALU for the base and four for the tenth power, against ten for
@@ -824,6 +824,10 @@ __device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& ho
 // smaller of the two applies: g = (U_k . (1, e_l)) min(q_l, r_k p_l), five
 // VALU for the base.  p, q here are exp(+-A l / n2), and 1/2^n2 is folded into
 // the P and Q records.  k holds U_k (0..2), L_k (3..5) and r_k (6).
+//
+// (Taking the bond's own radial factor out of the block sums, two VALU for the
+// base, was measured 2-3 times less accurate in FP32 where the diagonal blocks
+// then form their base in the scaled units; the form above is kept.)
 template <typename R, int N2_IS_10, int FORM>
 __device__ __forceinline__ R weight_rf(const R* k, R exl, R eyl, R pl, R ql, R n2)
 {
@@ -838,7 +842,7 @@ __device__ __forceinline__ R weight_rf(const R* k, R exl, R eyl, R pl, R ql, R n
     return mw_pow<N2_IS_10>(g, n2);
 }
 
-// The k-side factors of the rank form from a bond's record values.
+// The k-side factors of the rank form from a bond's geometry.
 template <typename R>
 __device__ __forceinline__ void rf_kside(R ex, R ey, R pe, R qe, R* k)
 {
@@ -852,13 +856,9 @@ __device__ __forceinline__ void rf_kside(R ex, R ey, R pe, R qe, R* k)
 }
 
 // k-side operands of the FP64 matrix-core contractions for one family: the
-// bonds of record rows 16 s + lk (s < NS).  Classic weights keep the half unit
-// vector and the radial factors; the rank form keeps U = p (1, e) and
-// L = q (1, e), and picks its form from the neighbour block q / 4 (neighbour
-// rows 4 q .. 4 q + 3) against s, both known at compile time once the loops
-// are unrolled.
-template <int NS, bool RFK> struct MfK {
-    double v[NS][RFK ? 7 : 4];
+// half unit vector and the radial factors of the bonds 16 s + lk (s < NS).
+template <int NS> struct MfK {
+    double v[NS][4];
 
     template <typename R, int FPW>
     __device__ __forceinline__ void load(const R* rec, int f, int lk)
@@ -866,32 +866,17 @@ template <int NS, bool RFK> struct MfK {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int kb = 16 * s + lk;
-            const double ex = rec[rec_at<R, FPW>(kb, f, 0)];
-            const double ey = rec[rec_at<R, FPW>(kb, f, 1)];
-            const double pe = rec[rec_at<R, FPW>(kb, f, 2)];
-            const double qe = rec[rec_at<R, FPW>(kb, f, 3)];
-            if constexpr (RFK) {
-                rf_kside(ex, ey, pe, qe, v[s]);
-            } else {
-                v[s][0] = 0.5 * ex;
-                v[s][1] = 0.5 * ey;
-                v[s][2] = pe;
-                v[s][3] = qe;
-            }
+            v[s][0] = 0.5 * rec[rec_at<R, FPW>(kb, f, 0)];
+            v[s][1] = 0.5 * rec[rec_at<R, FPW>(kb, f, 1)];
+            v[s][2] = rec[rec_at<R, FPW>(kb, f, 2)];
+            v[s][3] = rec[rec_at<R, FPW>(kb, f, 3)];
         }
     }
 
     template <int N2_IS_10>
-    __device__ __forceinline__ double w(int s, int q, double ex, double ey, double pe, double qe, double n2) const
+    __device__ __forceinline__ double w(int s, double ex, double ey, double pe, double qe, double n2) const
     {
-        if constexpr (RFK) {
-            const int bl = q >> 2;
-            if (bl > s) return weight_rf<double, N2_IS_10, 1>(v[s], ex, ey, pe, qe, n2);
-            if (bl < s) return weight_rf<double, N2_IS_10, -1>(v[s], ex, ey, pe, qe, n2);
-            return weight_rf<double, N2_IS_10, 0>(v[s], ex, ey, pe, qe, n2);
-        } else {
-            return weight<double, N2_IS_10>(v[s][0], v[s][1], v[s][2], v[s][3], ex, ey, pe, qe, n2);
-        }
+        return weight<double, N2_IS_10>(v[s][0], v[s][1], v[s][2], v[s][3], ex, ey, pe, qe, n2);
     }
 };
 
@@ -944,9 +929,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     constexpr bool RF = Contraction<R, MF>::RF;
     static_assert(!(MF64 || MW) || (Tile<LPF, BPL>::FS % 16 == 0),
                   "matrix-core contractions: families of a multiple of 16 slots");
-    static_assert(!RF || MF64 || LPF == 16, "rank-form VALU / FP32 contractions: 16 lanes per family");
-    // rank-form weights with the lane-per-rank-slot contractions (MF = 3, 4)
-    constexpr bool RFL = RF && !MF64;
+    static_assert(!RF || LPF == 16, "rank-form weights: 16 lanes per family");
     typedef typename Vec2<R>::T R2;
     constexpr int FS = Tile<LPF, BPL>::FS;
     constexpr int FPW = Tile<LPF, BPL>::FPW;
@@ -968,11 +951,13 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     }
     if (!FIX && lane == 0 && nnew) atomicAdd(a.counters, (unsigned int)nnew);
 
-    // ---- K_ik^-1: stored, or re-assembled in FP64 when the family changed ----
-    double kinv[BPL][3];
+    // ---- K_ik^-1: stored, or re-assembled in FP64 when the family changed;
+    // used in the bond arithmetic's precision ----
+    R kin[BPL][3];
     bool guard = false;   // mixed precision: the family is evaluated again in FP64
     if constexpr (FamLoad<R, BPL>::GUARD) guard = L.ff != 0;
     if (nnew != 0 || a.force_k) {
+        double kinv[BPL][3];
         Geo<double> g64[BPL];
 #pragma unroll
         for (int s = 0; s < BPL; ++s) g64[s] = geometry64(a, S[s], a.A);
@@ -995,13 +980,15 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                 if (F.famok && F.t == 0 && guard != (L.ff != 0)) a.famflag[F.fam] = guard ? 1 : 0;
             }
         }
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) kin[s][c] = R(kinv[s][c]);
     } else {
 #pragma unroll
-        for (int s = 0; s < BPL; ++s) {
-            kinv[s][0] = S[s].act ? double(L.kraw[s][0]) : 0.0;
-            kinv[s][1] = S[s].act ? double(L.kraw[s][1]) : 0.0;
-            kinv[s][2] = S[s].act ? double(L.kraw[s][2]) : 0.0;
-        }
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) kin[s][c] = S[s].act ? L.kraw[s][c] : R(0);
     }
 
     if constexpr (FamLoad<R, BPL>::GUARD) {
@@ -1019,21 +1006,15 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     constexpr bool COMPACT = sizeof(R) == 8 && !MF && BPL == 1;
     R* rec = reinterpret_cast<R*>(__builtin_assume_aligned(rec64, 16));
     Geo<R> g[BPL];
-    // MW records: rank slot of each of the lane's bonds (inactive slots keep
-    // their own position, past the family's ranks), and the 2^-n2 of (g/2)^n2
-    int mws[BPL];
+    // the 2^-n2 of (g/2)^n2 in the matrix-core and rank-form records
     const R mw_scale = N2_IS_10 ? R(1.0 / 1024.0) : R(exp2(-a.n2));
-    if constexpr (MW || RF) {
-#pragma unroll
-        for (int s = 0; s < BPL; ++s) mws[s] = S[s].act ? int(L.rk[s]) : S[s].k;
-    }
     if constexpr (MW) {
         typedef MWL<R, LPF, BPL> M;
 #pragma unroll
         for (int s = 0; s < BPL; ++s) {
             R U[3], Lq[3];
             mw_geometry<R, N2_IS_10>(a, S[s], U, Lq);
-            R* r0 = rec + M::at(F.slot, 0, mws[s]);
+            R* r0 = rec + M::at(F.slot, 0, S[s].k);
             r0[0] = U[0];
             r0[M::row(1)] = U[1];
             r0[M::row(2)] = U[2];
@@ -1049,13 +1030,17 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             r0[M::row(9)] = Vd1 * x1r;
         }
     }
+    // rank form: the lane's own bonds are its slots (families in length
+    // order); their k-side factors U = p (1, e), L = q (1, e), r = q / p
+    R Kk[RF ? BPL : 1][7];
     if constexpr (!MW) {
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
         // rank-form records carry exp(+-A l / n2) and the P scaled by 2^-n2
         g[s] = GeoOf<R>::get(a, S[s], RF ? (N2_IS_10 ? a.A * 0.1 : a.A / a.n2) : a.A);
-        const int k = RF ? mws[s] : S[s].k;
+        const int k = S[s].k;
         rec_put4<FPW>(rec, k, F.slot, 0, g[s].ex, g[s].ey, g[s].pe, g[s].qe);
+        if constexpr (RF) rf_kside(g[s].ex, g[s].ey, g[s].pe, g[s].qe, Kk[s]);
         if constexpr (COMPACT) {
             const double Vl = alive[s] ? S[s].Vj * double(g[s].len) : 0.0;
             rec_put<FPW>(rec, k, F.slot, 2, R(Vl * S[s].d0), R(Vl * S[s].d1));
@@ -1072,20 +1057,9 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     // run to the wave's largest family rounded up to 4 without guards
     const int mmax = wave_max_family<LPF>(F.M);
     wave_sync();
-    // rank form: lane t works on the rank slots t + 16 s in the contractions;
-    // its k-side factors U = p (1, e), L = q (1, e), r = q / p of those slots
-    R Kk[RFL ? BPL : 1][7];
-    if constexpr (RFL) {
-#pragma unroll
-        for (int s = 0; s < BPL; ++s) {
-            const int kr = F.t + 16 * s;
-            rf_kside(rec[rec_at<R, FPW>(kr, F.slot, 0)], rec[rec_at<R, FPW>(kr, F.slot, 1)],
-                     rec[rec_at<R, FPW>(kr, F.slot, 2)], rec[rec_at<R, FPW>(kr, F.slot, 3)], Kk[s]);
-        }
-    }
     // the weight of slot s's bond against record l
     auto wt = [&](int s, int l, R ex, R ey, R pe, R qe) -> R {
-        if constexpr (RFL) {
+        if constexpr (RF) {
             const int bl = l >> 4;
             if (bl > s) return weight_rf<R, N2_IS_10, 1>(Kk[s], ex, ey, pe, qe, R(a.n2));
             if (bl < s) return weight_rf<R, N2_IS_10, -1>(Kk[s], ex, ey, pe, qe, R(a.n2));
@@ -1142,13 +1116,13 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) D[s][c] = rec[MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW + c, mws[s])];
+            for (int c = 0; c < 4; ++c) D[s][c] = rec[MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW + c, S[s].k)];
     } else if constexpr (MF64) {
         double Dacc[WF][NS];
 #pragma unroll
         for (int f = 0; f < WF; ++f) {
             Mf[f] = __builtin_amdgcn_readfirstlane(__shfl(F.M, f * LPF, 64));
-            MfK<NS, RF> ks;
+            MfK<NS> ks;
             ks.template load<R, FPW>(rec, f, lk);
 #pragma unroll
             for (int s = 0; s < NS; ++s) Dacc[f][s] = 0.0;
@@ -1165,7 +1139,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                     // between the independent weight chains
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
-                        const double om = ks.template w<N2_IS_10>(s, q, r.ex, r.ey, r.pe, r.qe, a.n2);
+                        const double om = ks.template w<N2_IS_10>(s, r.ex, r.ey, r.pe, r.qe, a.n2);
                         if constexpr (!MFRW) wc[f][q][s] = om;
                         Dacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(r.y, om, Dacc[f][s], 0, 0, 0);
                     }
@@ -1188,7 +1162,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(RF ? mws[s] : S[s].k, F.slot, 4 + c)];
+            for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
     } else {
 #pragma unroll
     for (int l0 = 0; l0 < FS; l0 += GL) {
@@ -1256,17 +1230,6 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             for (int c = 0; c < 4; ++c) D[s][c] = Dv[s][c];
     }
     }
-    if constexpr (RFL) {
-        // D of the rank slots back to the lanes' own bonds, through the P rows
-        wave_sync();
-#pragma unroll
-        for (int s = 0; s < BPL; ++s) rec_put4<FPW>(rec, F.t + 16 * s, F.slot, 1, D[s][0], D[s][1], D[s][2], D[s][3]);
-        wave_sync();
-#pragma unroll
-        for (int s = 0; s < BPL; ++s)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(mws[s], F.slot, 4 + c)];
-    }
 
     // ---- F - I = D K^-1, plane-stress sigma, Q_k = V_k sigma K^-1 ----
     const double pf = F.pf;
@@ -1281,7 +1244,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     if constexpr (FamLoad<R, BPL>::GUARD) dmg_here = !(a.famlist && guard);
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
-        const R K00 = R(kinv[s][0]), K01 = R(kinv[s][1]), K11 = R(kinv[s][2]);
+        const R K00 = kin[s][0], K01 = kin[s][1], K11 = kin[s][2];
         const R G00 = D[s][0] * K00 + D[s][1] * K01;
         const R G01 = D[s][0] * K01 + D[s][1] * K11;
         const R G10 = D[s][2] * K00 + D[s][3] * K01;
@@ -1310,7 +1273,8 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
         const R s11 = c11 * e11 + c12 * e22;
         const R s22 = c12 * e11 + c11 * e22;
         const R s12 = c33 * e12;
-        const R Vk = alive[s] ? R(S[s].Vj) : R(0);
+        // the matrix-core and rank-form records carry Q scaled by 2^-n2
+        const R Vk = alive[s] ? R(S[s].Vj) * ((MW || RF) ? mw_scale : R(1)) : R(0);
         Q[s][0] = (s11 * K00 + s12 * K01) * Vk;
         Q[s][1] = (s11 * K01 + s12 * K11) * Vk;
         Q[s][2] = (s12 * K00 + s22 * K01) * Vk;
@@ -1321,12 +1285,11 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     for (int s = 0; s < BPL; ++s) {
         if constexpr (MW) {
             constexpr int PSPm = MWL<R, LPF, BPL>::PSP;
-            R* r8 = rec + MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW, mws[s]);
+            R* r8 = rec + MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW, S[s].k);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) r8[c * PSPm] = Q[s][c] * mw_scale;
+            for (int c = 0; c < 4; ++c) r8[c * PSPm] = Q[s][c];
         } else if constexpr (RF) {
-            rec_put4<FPW>(rec, mws[s], F.slot, 1, Q[s][0] * mw_scale, Q[s][1] * mw_scale, Q[s][2] * mw_scale,
-                          Q[s][3] * mw_scale);
+            rec_put4<FPW>(rec, S[s].k, F.slot, 1, Q[s][0], Q[s][1], Q[s][2], Q[s][3]);
         } else {
             const int k = S[s].k;
             rec_put4<FPW>(rec, k, F.slot, 1, Q[s][0], Q[s][1], Q[s][2], Q[s][3]);
@@ -1344,13 +1307,13 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW + c, mws[s])];
+            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[MWL<R, LPF, BPL>::at(F.slot, MWL<R, LPF, BPL>::PROW + c, S[s].k)];
     } else if constexpr (MF64) {
         double Sacc[WF][NS];
 #pragma unroll
         for (int f = 0; f < WF; ++f) {
             // the records' geometry values (0..3) are still in place
-            MfK<NS, RF> ks;
+            MfK<NS> ks;
             if constexpr (MFRW) ks.template load<R, FPW>(rec, f, lk);
 #pragma unroll
             for (int s = 0; s < NS; ++s) Sacc[f][s] = 0.0;
@@ -1365,7 +1328,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                     for (int s = 0; s < NS; ++s) {
                         double om;
                         if constexpr (MFRW)
-                            om = ks.template w<N2_IS_10>(s, q, r.ex, r.ey, r.pe, r.qe, a.n2);
+                            om = ks.template w<N2_IS_10>(s, r.ex, r.ey, r.pe, r.qe, a.n2);
                         else
                             om = wc[f][q][s];
                         Sacc[f][s] = __builtin_amdgcn_mfma_f64_4x4x4f64(r.y, om, Sacc[f][s], 0, 0, 0);
@@ -1383,7 +1346,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(RF ? mws[s] : S[s].k, F.slot, 4 + c)];
+            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
     } else {
     v4f Sv[BPL];
 #pragma unroll
@@ -1393,15 +1356,15 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
         if (l0 < mmax) {
 #pragma unroll
             for (int l = l0; l < l0 + GL; ++l) {
+                R ex = R(0), ey = R(0), pe = R(0), qe = R(0);
+                if constexpr (RW) {
+                    ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
+                    ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
+                    pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
+                    qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
+                }
                 if constexpr (MF32) {
                     const R Qa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
-                    R ex = R(0), ey = R(0), pe = R(0), qe = R(0);
-                    if constexpr (RW) {
-                        ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
-                        ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
-                        pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
-                        qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
-                    }
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         R om;
@@ -1416,13 +1379,6 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
                     const R Q1 = rec[rec_at<R, FPW>(l, F.slot, 5)];
                     const R Q2 = rec[rec_at<R, FPW>(l, F.slot, 6)];
                     const R Q3 = rec[rec_at<R, FPW>(l, F.slot, 7)];
-                    R ex = R(0), ey = R(0), pe = R(0), qe = R(0);
-                    if constexpr (RW) {
-                        ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
-                        ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
-                        pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
-                        qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
-                    }
 #pragma unroll
                     for (int s = 0; s < BPL; ++s) {
                         R om;
@@ -1446,18 +1402,6 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
             for (int c = 0; c < 4; ++c) Sm[s][c] = Sv[s][c];
     }
-    }
-    if constexpr (RFL) {
-        // S of the rank slots back to the lanes' own bonds
-        wave_sync();
-#pragma unroll
-        for (int s = 0; s < BPL; ++s)
-            rec_put4<FPW>(rec, F.t + 16 * s, F.slot, 1, Sm[s][0], Sm[s][1], Sm[s][2], Sm[s][3]);
-        wave_sync();
-#pragma unroll
-        for (int s = 0; s < BPL; ++s)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(mws[s], F.slot, 4 + c)];
     }
     // V_H and the own-bond force sum accumulate in the bond arithmetic's
     // precision (FP32 in mixed precision, like the T they sum; measured -2 %)
@@ -1520,7 +1464,7 @@ __global__ void __launch_bounds__(512) k_bond(BondArgs a)
     const int blk = xcd_remap(blockIdx.x, gridDim.x, a.xcd);
     FamLoad<R, BPL> L;
     stage_rows<LPF>(a, L, lane, blk * wpb + wib);
-    stage_cols<R, LPF, BPL, DMG, 0, MF >= 2>(a, L);
+    stage_cols<R, LPF, BPL, DMG>(a, L);
     stage_bonds<R, LPF, BPL>(a, L);
     bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(a, L, smem + wib * WaveLds<R, LPF, BPL, MF>::DOUBLES, lane,
                                               NoHook(), NoHook());
@@ -1549,7 +1493,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
     if (g >= g1) return;
     FamLoad<R, BPL> cur, nxt;
     stage_rows<LPF>(a, cur, lane, g);
-    stage_cols<R, LPF, BPL, DMG, 0, MF >= 2>(a, cur);
+    stage_cols<R, LPF, BPL, DMG>(a, cur);
     stage_bonds<R, LPF, BPL>(a, cur);
     for (;;) {
         const int gn = g + wpb;
@@ -1561,7 +1505,7 @@ __global__ void __launch_bounds__(512) k_bond_p(BondArgs a)
         stage_rows<LPF>(a, nxt, lane, gp);
         bond_body<R, LPF, BPL, N2_IS_10, MF, DMG>(
             a, cur, rec64, lane,
-            [&]() { stage_cols<R, LPF, BPL, DMG, 0, MF >= 2>(a, nxt); },
+            [&]() { stage_cols<R, LPF, BPL, DMG>(a, nxt); },
             [&]() { stage_bonds<R, LPF, BPL>(a, nxt); });
         if (!more) break;
         cur = nxt;

@@ -46,7 +46,6 @@ struct pd_context {
     int *row = nullptr, *col = nullptr, *rev = nullptr, *bc = nullptr, *ids = nullptr;
     long ids_cap = 0;
     unsigned char* broken = nullptr;
-    unsigned char* rank = nullptr;      // per bond: length rank in its family (k_ranks)
     void* W = nullptr;
     unsigned int* counters = nullptr;
     double2* stage = nullptr;           // pinned host staging for one particle vector

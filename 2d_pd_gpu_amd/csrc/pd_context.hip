@@ -2,9 +2,10 @@
 //
 // The caller's particles are reordered along a Morton curve at creation so that
 // a family's neighbours sit in nearby cache lines; bonds stay grouped per
-// family in the caller's within-family order.  All state lives in device
-// memory for the context's lifetime; host arrays cross the ABI only in the
-// caller's order (the permutation never leaks).
+// family, in ascending order of their reference lengths (ties in the caller's
+// within-family order), the order the rank-form weights rely on.  All state
+// lives in device memory for the context's lifetime; host arrays cross the ABI
+// only in the caller's order (the permutations never leak).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,7 +30,7 @@ static void free_all(pd_context* c)
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
     void* ptrs[] = {c->X, c->u, c->v, c->a, c->f, c->fown, c->v0, c->V, c->E, c->pf1, c->kinv,
-                    c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->rank, c->W, c->counters,
+                    c->row, c->col, c->rev, c->bc, c->ids, c->broken, c->W, c->counters,
                     c->kappa, c->dmg, c->dstat, c->famflag, c->famlist, c->famcnt};
     for (void* q : ptrs)
         if (q) hipFree(q);
@@ -50,7 +51,6 @@ static BondArgs bond_args(pd_context* c)
     a.row = c->row;
     a.col = c->col;
     a.rev = c->rev;
-    a.rank = c->rank;
     a.u = c->u;
     a.broken = c->broken;
     a.kinv = c->kinv;
@@ -254,7 +254,6 @@ static int finish_create(pd_context* c)
     if ((rc = dalloc(c, &c->u, n)) || (rc = dalloc(c, &c->v, n)) || (rc = dalloc(c, &c->a, n)) ||
         (rc = dalloc(c, &c->f, n)) || (rc = dalloc(c, &c->fown, n)) ||
         (rc = dalloc(c, (char**)&c->kinv, kinv_bytes(c))) || (rc = dalloc(c, &c->broken, nb)) ||
-        (rc = dalloc(c, &c->rank, nb)) ||
         (rc = dalloc(c, &c->pf1, 1)) || (rc = dalloc(c, (char**)&c->W, wbytes * (size_t)(nb > 0 ? nb : 1))) ||
         (rc = dalloc(c, &c->counters, 4)) || (rc = dalloc(c, &c->bc, n)) || (rc = dalloc(c, &c->v0, 1)))
         return rc;
@@ -293,8 +292,6 @@ static int finish_create(pd_context* c)
         std::vector<int> code(n, PD_BC_FREE);
         e = e != hipSuccess ? e : hipMemcpy(c->bc, code.data(), sizeof(int) * n, hipMemcpyHostToDevice);
     }
-    if (nb == 0) e = e != hipSuccess ? e : hipMemset(c->rank, 0, 1);
-    e = e != hipSuccess ? e : launch_ranks(n, c->row, c->col, c->X, c->rank, 0);
     e = e != hipSuccess ? e : hipDeviceSynchronize();
     if (e != hipSuccess) return fail(c, PD_ERR_HIP, "initialisation failed: %s", hipGetErrorString(e));
     return run_shape(c);
@@ -367,10 +364,28 @@ extern "C" int pd_create(const pd_params* p, int n, const double* X, const doubl
         row[q + 1] = row[q] + (row_ptr[i + 1] - row_ptr[i]);
     }
     parallel_for(n, [&](long q0, long q1) {
+        std::vector<int> ord;
+        std::vector<double> len2;
         for (long q = q0; q < q1; ++q) {
-            int i = c->perm[q];
+            const int i = c->perm[q];
+            const int m = row_ptr[i + 1] - row_ptr[i];
+            // the family in ascending order of |X_j - X_i|^2 (each operation
+            // rounded on its own, as k_length_order does), ties in the
+            // caller's order
+            len2.resize(m);
+            ord.resize(m);
+            for (int e = 0; e < m; ++e) {
+                const int j = col[row_ptr[i] + e];
+                const double d0 = X[2 * (size_t)j] - X[2 * (size_t)i];
+                const double d1 = X[2 * (size_t)j + 1] - X[2 * (size_t)i + 1];
+                const double s0 = d0 * d0, s1 = d1 * d1;
+                len2[e] = s0 + s1;
+                ord[e] = e;
+            }
+            std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return len2[x] < len2[y]; });
             long bi = row[q];
-            for (int e = row_ptr[i]; e < row_ptr[i + 1]; ++e, ++bi) {
+            for (int k = 0; k < m; ++k, ++bi) {
+                const int e = row_ptr[i] + ord[k];
                 coli[bi] = c->iperm[col[e]];
                 c->bmap[bi] = e;
             }
@@ -576,7 +591,7 @@ extern "C" int pd_set_option(pd_context* c, int opt, long value)
         }
         if (mf >= 2 && !(c->p.n2 > 0))
             return fail(c, PD_ERR_INVALID, "matrix-core and rank-form weights need an angular exponent n2 > 0");
-        if (value < 0 || mf > 5 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
+        if (value < 0 || mf > 4 || !tile_supported(c->p.precision == PD_MIXED, lpf, bpl, mf) ||
             lpf * bpl < c->maxM)
             return fail(c, PD_ERR_INVALID, "tile %ld (lanes %d x bonds %d, matrix cores %d) unsupported for "
                         "this precision or smaller than the largest family (%d)", value, lpf, bpl, mf, c->maxM);
@@ -1169,7 +1184,7 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     std::vector<Var> vars;
     const int mixed = c->p.precision == PD_MIXED;
     const int tiles[5][2] = {{32, 1}, {64, 1}, {16, 2}, {32, 2}, {16, 3}};
-    for (int mf : {0, 1, 2, 3, 4, 5})
+    for (int mf : {0, 1, 2, 3, 4})
         for (const auto& tl : tiles) {
             if (!tile_supported(mixed, tl[0], tl[1], mf) || tl[0] * tl[1] < c->maxM) continue;
             if (mf >= 2 && !(c->p.n2 > 0)) continue;

@@ -9,7 +9,8 @@ namespace pd {
 
 // Everything the bond kernel needs for one launch.  Particle arrays are in the
 // internal (space-filling-curve) order; bond arrays in the internal CSR order,
-// one family (the bonds of particle i) contiguous.
+// one family (the bonds of particle i) contiguous and in ascending order of
+// the bonds' reference lengths |X_j - X_i| (ties in the caller's order).
 struct BondArgs {
     int n;                    // families (= particles)
     int xcd;                  // XCDs to spread consecutive blocks over (<= 1: off)
@@ -18,7 +19,6 @@ struct BondArgs {
     const int* row;           // CSR row pointer, n+1
     const int* col;           // neighbour of each bond
     const int* rev;           // reverse bond of each bond
-    const unsigned char* rank; // per bond: rank of |xi| in its family (ties by position)
     const double2* u;         // displacement y - X
     unsigned char* broken;    // per bond, 1 = broken (irreversible)
     void* kinv;               // per bond K_ik^-1 (xx, xy, yy), assembled and inverted in
@@ -108,10 +108,32 @@ hipError_t launch_velocity_bc_list(int count, const int* ids, double2 v0, double
                                    double2* a, hipStream_t s);
 hipError_t launch_fixed_bc_list(int count, const int* ids, double2* u, hipStream_t s);
 hipError_t launch_fill_u32(unsigned int* p, unsigned int v, int count, hipStream_t s);
-// rank[b] = rank of bond b's reference length |X_j - X_i| within its family
-// (ties by position): the record order of the matrix-core-weight kernels.
-hipError_t launch_ranks(int n, const int* row, const int* col, const double2* X, unsigned char* rank,
-                        hipStream_t s);
+// Each family's (col, bmap) entries in ascending order of |X_j - X_i|^2,
+// ties in their present order (the order of the families' bond storage).
+hipError_t launch_length_order(int n, const int* row, const double2* X, int* col, int* bmap, hipStream_t s);
+
+// The length order of one family of m <= 64 bonds (neighbours cj, the
+// particle's position xi): ord[r] = the family position of the bond of rank r,
+// ascending |X_j - X_i|^2 with ties in position order.  Each operation is
+// rounded on its own, as in the host set-up.  Neighbour indices outside
+// [0, n) sort as zero length (callers reject such meshes).
+__device__ inline void family_length_order(int m, const int* cj, const double2* X, int n, double2 xi, int* ord)
+{
+#pragma clang fp contract(off)
+    double len2[64];
+    for (int e = 0; e < m; ++e) {
+        const int j = cj[e];
+        const double2 xj = (j >= 0 && j < n) ? X[j] : xi;
+        const double d0 = xj.x - xi.x, d1 = xj.y - xi.y;
+        const double s0 = d0 * d0, s1 = d1 * d1;
+        len2[e] = s0 + s1;
+    }
+    for (int e = 0; e < m; ++e) {
+        int r = 0;
+        for (int f = 0; f < m; ++f) r += (len2[f] < len2[e] || (len2[f] == len2[e] && f < e)) ? 1 : 0;
+        ord[r] = e;
+    }
+}
 hipError_t launch_bond_fix(const BondArgs& a, int n2_is_10, int dmg, int grid, hipStream_t s);
 hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32, const unsigned char* broken,
                                        double cmin, unsigned int* count, hipStream_t s);

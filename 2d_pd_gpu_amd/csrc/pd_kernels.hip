@@ -188,24 +188,24 @@ __global__ void __launch_bounds__(256) k_count_ill_conditioned(long nb, const KT
 
 // Rank of each bond's reference length in its family (squared lengths in
 // FP64, ties broken by position), one thread per family.
-__global__ void __launch_bounds__(256) k_ranks(int n, const int* __restrict__ row, const int* __restrict__ col,
-                                               const double2* __restrict__ X, unsigned char* __restrict__ rank)
+// Families in ascending order of |X_j - X_i|^2, ties in their present order:
+// one thread per family reorders its (col, bmap) entries in place.
+__global__ void __launch_bounds__(256) k_length_order(int n, const int* __restrict__ row,
+                                                      const double2* __restrict__ X, int* __restrict__ col,
+                                                      int* __restrict__ bmap)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int b0 = row[i], m = row[i + 1] - b0;
-    const double2 xi = X[i];
-    double len2[64];
-    for (int e = 0; e < m && e < 64; ++e) {
-        const double2 xj = X[col[b0 + e]];
-        const double d0 = xj.x - xi.x, d1 = xj.y - xi.y;
-        len2[e] = d0 * d0 + d1 * d1;
+    const int b0 = row[i], m = min(row[i + 1] - b0, 64);
+    int cj[64], bm[64], ord[64];
+    for (int e = 0; e < m; ++e) {
+        cj[e] = col[b0 + e];
+        bm[e] = bmap[b0 + e];
     }
-    for (int e = 0; e < m && e < 64; ++e) {
-        int r = 0;
-        for (int f = 0; f < m && f < 64; ++f)
-            r += (len2[f] < len2[e] || (len2[f] == len2[e] && f < e)) ? 1 : 0;
-        rank[b0 + e] = (unsigned char)r;
+    family_length_order(m, cj, X, n, X[i], ord);
+    for (int r = 0; r < m; ++r) {
+        col[b0 + r] = cj[ord[r]];
+        bmap[b0 + r] = bm[ord[r]];
     }
 }
 
@@ -234,7 +234,6 @@ bool tile_supported(int mixed, int lpf, int bpl, int mf)
     if (mf == 2) return !mixed && ((bpl == 1 && (lpf == 32 || lpf == 64)) || (lpf == 16 && bpl == 3));
     if (mf == 3) return lpf == 16 && (bpl == 2 || bpl == 3);
     if (mf == 4) return mixed && lpf == 16 && (bpl == 2 || bpl == 3);
-    if (mf == 5) return !mixed && ((bpl == 1 && (lpf == 32 || lpf == 64)) || (lpf == 16 && (bpl == 2 || bpl == 3)));
     if (mf && !mixed) return (bpl == 1 && (lpf == 32 || lpf == 64)) || (lpf == 16 && (bpl == 2 || bpl == 3));
     if (bpl == 1) return lpf == 32 || lpf == 64;
     if (bpl == 2) return lpf == 16 || (lpf == 32 && mixed);
@@ -346,11 +345,10 @@ hipError_t launch_count_ill_conditioned(long nb, const void* kinv, bool kinv_f32
     return hipGetLastError();
 }
 
-hipError_t launch_ranks(int n, const int* row, const int* col, const double2* X, unsigned char* rank,
-                        hipStream_t s)
+hipError_t launch_length_order(int n, const int* row, const double2* X, int* col, int* bmap, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ranks, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, row, col, X, rank);
+    hipLaunchKernelGGL(k_length_order, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, row, X, col, bmap);
     return hipGetLastError();
 }
 

@@ -289,6 +289,7 @@ int build_mesh_on_device(pd_context* c, const double* Xh, const double* Vh)
     hipLaunchKernelGGL(k_internal_csr, dim3(grid_of(n)), dim3(256), 0, s, n, perm, iperm, rowc, colc, c->row,
                        c->col, bmap);
     SETCHK(hipGetLastError());
+    SETCHK(pd::launch_length_order(n, c->row, c->X, c->col, bmap, s));
     SETCHK(hipMemsetAsync(missing, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_reverse, dim3(grid_of(n)), dim3(256), 0, s, n, c->row, c->col, c->rev, missing);
     SETCHK(hipGetLastError());

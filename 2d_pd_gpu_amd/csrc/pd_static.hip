@@ -6,13 +6,18 @@
 //
 // The arithmetic is pd_compute_velocity's on a context of the same mesh: the
 // same bond kernel (tile (32,1) or (64,1), VALU contractions), K^-1 assembled
-// by the same shape kernel, the same gather.  Particles stay in the caller's
-// order (no space-filling-curve reorder), which changes no family sum.
+// by the same shape kernel, the same gather, and each family's bonds taken in
+// the same ascending order of their reference lengths.  Particles stay in the
+// caller's order (no space-filling-curve reorder), which changes no family
+// sum.
 //
 // Workspace (`scratch`, one copy per GPU, held until the process ends):
-//   rev   int32   per bond      reverse-bond index
+//   rev   int32   per bond      reverse-bond index (length-ordered families)
 //   kinv  double3 per bond      K^-1
 //   W     double2 per bond      reverse-slot force terms
+//   col   int32   per bond      neighbours, each family in length order
+//   bmap  int32   per bond      the caller's bond of each length-ordered bond
+//   brk   uint8   per bond      broken flags in the length order
 //   fown  double2 per particle  own-bond force sums
 //   misc  counters (4 x uint32) and the plane-stress factor
 // Every call on a GPU uses that GPU's workspace: calls that overlap on one GPU
@@ -33,10 +38,14 @@ constexpr long kMaxBonds = PD_STATIC_MAX_BONDS;
 constexpr size_t kRevOff = 0;
 constexpr size_t kKinvOff = kRevOff + 4 * (size_t)kMaxBonds;
 constexpr size_t kWOff = kKinvOff + 24 * (size_t)kMaxBonds;
-constexpr size_t kFownOff = kWOff + 16 * (size_t)kMaxBonds;
+constexpr size_t kColOff = kWOff + 16 * (size_t)kMaxBonds;
+constexpr size_t kBmapOff = kColOff + 4 * (size_t)kMaxBonds;
+constexpr size_t kBrkOff = kBmapOff + 4 * (size_t)kMaxBonds;
+constexpr size_t kFownOff = kBrkOff + (size_t)kMaxBonds;
 constexpr size_t kMiscOff = kFownOff + 16 * (size_t)kMaxParticles;
 constexpr size_t kScratchBytes = kMiscOff + 256;
-static_assert(kKinvOff % 256 == 0 && kWOff % 256 == 0 && kFownOff % 256 == 0 && kMiscOff % 256 == 0,
+static_assert(kKinvOff % 256 == 0 && kWOff % 256 == 0 && kColOff % 256 == 0 && kBmapOff % 256 == 0 &&
+                  kBrkOff % 256 == 0 && kFownOff % 256 == 0 && kMiscOff % 256 == 0,
               "workspace sections stay 256-byte aligned");
 
 // The workspace.  In host code `scratch` names a host-side stand-in, not
@@ -67,6 +76,39 @@ int workspace(int dev, unsigned char** out)
     return PD_OK;
 }
 
+// Per particle i: its family in ascending order of the bonds' reference
+// lengths (ties in the caller's order): the neighbours, the caller's bond of
+// each, and its broken flag.  Families of more than 64 bonds are left to the
+// mesh check.
+__global__ void __launch_bounds__(256) k_static_order(int n, const int* __restrict__ row,
+                                                      const int* __restrict__ col,
+                                                      const unsigned char* __restrict__ broken,
+                                                      const double2* __restrict__ X, int* __restrict__ col_o,
+                                                      int* __restrict__ bmap, unsigned char* __restrict__ brk_o)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b0 = row[i], m = row[i + 1] - b0;
+    if (m > 64) return;
+    int cj[64], ord[64];
+    for (int e = 0; e < m; ++e) cj[e] = col[b0 + e];
+    family_length_order(m, cj, X, n, X[i], ord);
+    for (int r = 0; r < m; ++r) {
+        col_o[b0 + r] = cj[ord[r]];
+        bmap[b0 + r] = b0 + ord[r];
+        brk_o[b0 + r] = broken[b0 + ord[r]];
+    }
+}
+
+// The broken flags back in the caller's order.
+__global__ void __launch_bounds__(256) k_static_broken_out(long nb, const int* __restrict__ bmap,
+                                                           const unsigned char* __restrict__ brk_o,
+                                                           unsigned char* __restrict__ broken)
+{
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) broken[bmap[b]] = brk_o[b];
+}
+
 // Per particle i: the reverse bond of each of its bonds (i -> j finds i in
 // row j), the mesh checks (neighbour in range and not i, a reverse found,
 // found once), and, by thread 0, the counters cleared and the plane-stress
@@ -75,7 +117,7 @@ int workspace(int dev, unsigned char** out)
 __global__ void __launch_bounds__(256) k_static_prep(int n, const int* __restrict__ row,
                                                      const int* __restrict__ col, int* __restrict__ rev,
                                                      unsigned int* __restrict__ counters, double* pf,
-                                                     double pf_value)
+                                                     double pf_value, int max_family)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
@@ -85,7 +127,7 @@ __global__ void __launch_bounds__(256) k_static_prep(int n, const int* __restric
         *pf = pf_value;
     }
     if (i >= n) return;
-    bool bad = false;
+    bool bad = row[i + 1] - row[i] > max_family;
     for (int b = row[i]; b < row[i + 1]; ++b) {
         const int j = col[b];
         int r = -1, hits = 0;
@@ -144,6 +186,9 @@ extern "C" int pd_compute_velocity_static(const pd_params* p, int n, long nb, in
     unsigned char* ws = nullptr;
     if (int rc = workspace(dev, &ws)) return rc;
     int* rev = reinterpret_cast<int*>(ws + kRevOff);
+    int* col_o = reinterpret_cast<int*>(ws + kColOff);
+    int* bmap = reinterpret_cast<int*>(ws + kBmapOff);
+    unsigned char* brk_o = ws + kBrkOff;
     unsigned int* counters = reinterpret_cast<unsigned int*>(ws + kMiscOff);
     double* pf = reinterpret_cast<double*>(ws + kMiscOff + 64);
 
@@ -153,8 +198,11 @@ extern "C" int pd_compute_velocity_static(const pd_params* p, int n, long nb, in
     if (hipStreamIsCapturing(s, &cap) != hipSuccess)
         return fail(nullptr, PD_ERR_HIP, "hipStreamIsCapturing failed");
     hipLaunchKernelGGL(k_static_clear_flag, dim3(1), dim3(1), 0, s, counters);
-    hipLaunchKernelGGL(k_static_prep, dim3((n + 255) / 256), dim3(256), 0, s, n, row_ptr, col, rev, counters,
-                       pf, p->E / (1.0 - p->nu * p->nu));
+    if (nb > 0)
+        hipLaunchKernelGGL(k_static_order, dim3((n + 255) / 256), dim3(256), 0, s, n, row_ptr, col, broken,
+                           reinterpret_cast<const double2*>(X), col_o, bmap, brk_o);
+    hipLaunchKernelGGL(k_static_prep, dim3((n + 255) / 256), dim3(256), 0, s, n, row_ptr, col_o, rev, counters,
+                       pf, p->E / (1.0 - p->nu * p->nu), max_family);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(nullptr, PD_ERR_HIP, "mesh check launch failed: %s", hipGetErrorString(e));
     if (cap == hipStreamCaptureStatusNone) {
@@ -174,10 +222,10 @@ extern "C" int pd_compute_velocity_static(const pd_params* p, int n, long nb, in
     ba.X = reinterpret_cast<const double2*>(X);
     ba.V = V;
     ba.row = row_ptr;
-    ba.col = nb > 0 ? col : rev;   // the bond kernel reads entry 0 for idle lanes
+    ba.col = nb > 0 ? col_o : rev;   // the bond kernel reads entry 0 for idle lanes
     ba.rev = rev;
     ba.u = reinterpret_cast<const double2*>(u);
-    ba.broken = nb > 0 ? broken : reinterpret_cast<unsigned char*>(ws + kMiscOff + 128);
+    ba.broken = nb > 0 ? brk_o : reinterpret_cast<unsigned char*>(ws + kMiscOff + 128);
     ba.kinv = ws + kKinvOff;
     ba.kinv_f32 = 0;
     ba.W = ws + kWOff;
@@ -198,6 +246,11 @@ extern "C" int pd_compute_velocity_static(const pd_params* p, int n, long nb, in
     e = launch_shape(ba, lpf, 1, n2_is_10, 4, s);
     ba.force_k = 0;
     e = e != hipSuccess ? e : launch_bond(ba, 0, lpf, 1, 0, 0, n2_is_10, 4, 0, s);
+    if (e == hipSuccess && nb > 0) {
+        hipLaunchKernelGGL(k_static_broken_out, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, nb, bmap,
+                           brk_o, broken);
+        e = hipGetLastError();
+    }
     GatherArgs g = {};
     g.n = n;
     g.xcd = 8;

@@ -73,8 +73,7 @@ extern "C" {
                                    23201, 26401, 21603 (PD_FP64, n2 > 0); rank-form
                                    weights 31602, 31603 (and 41602, 41603 with
                                    matrix-core contractions in mixed precision;
-                                   53201, 56401, 51602, 51603 with matrix-core
-                                   contractions in PD_FP64; n2 > 0);
+                                   n2 > 0);
                                    0 = 3201 or 6401 by the largest family            */
 #define PD_OPT_PERSISTENT    6  /* bond kernel as a persistent grid of value workgroups
                                    per CU that prefetch their next family group
@@ -301,7 +300,7 @@ int pd_probe_copy(long bytes, int iters, double* gbps);
  *   broken  uint8[nb], in/out (bonds that break are set to 1).
  * nb = row_ptr[n] and max_family (the largest family) are passed by the host.
  * Every intermediate (reverse bonds, K^-1, the force terms) lives in a fixed
- * per-GPU workspace inside the library (about 24 MiB, held until the process
+ * per-GPU workspace inside the library (about 29 MiB, held until the process
  * ends), so the call allocates nothing; it works for n <= PD_STATIC_MAX_PARTICLES
  * and nb <= PD_STATIC_MAX_BONDS and rejects larger meshes.  Work is enqueued on
  * `stream` (hipStream_t, NULL = the null stream).  Outside stream capture the

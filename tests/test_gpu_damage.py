@@ -112,7 +112,7 @@ def test_damage_during_steps_matches_oracle(pdpkg, plate, mode):
 
 @pytest.mark.parametrize("precision,tile", [(0, 13201), (0, 16401), (0, 11603), (1, 11602),
                                             (1, 13201), (0, 21603), (0, 23201), (0, 31603), (1, 41603),
-                                            (0, 51603), (0, 53201)])
+                                            (0, 31602)])
 def test_damage_with_matrix_core_contractions(pdpkg, plate, precision, tile):
     """The fused damage update on the matrix-core tiles, 300 steps against the
     oracle (mode 2, the published law)."""

@@ -274,8 +274,7 @@ def test_launch_shapes_and_autotune_bitwise(pdpkg, small_plate):
     pdh = pdpkg.pdhip
     ref, bref, _ = run_plate_gpu(pdh, small_plate, 0.005, 40)
     mask = small_plate.code >= 0
-    for tile in (3201, 6401, 1602, 1603, 13201, 16401, 11602, 11603, 23201, 26401, 21603, 31602, 31603,
-                 53201, 56401, 51602, 51603):
+    for tile in (3201, 6401, 1602, 1603, 13201, 16401, 11602, 11603, 23201, 26401, 21603, 31602, 31603):
         tile_ref = None
         for wpb, xcd, persist in ((1, 0, 0), (2, 1, 0), (8, 0, 0), (8, 1, 0), (4, 1, 1), (2, 0, 3),
                                   (8, 1, 2)):
@@ -342,7 +341,7 @@ def test_hassa_matrix_core_contractions(pdpkg, hassa_case, hassa_oracle):
     v_mfma_f32_4x4x1 in mixed precision) meet the same bars as the VALU path."""
     say("hassa single step, matrix-core contractions")
     u, v, a, f, T = hassa_oracle
-    for tile in (13201, 16401, 11602, 11603, 23201, 26401, 21603, 31602, 31603, 53201, 56401, 51602, 51603):
+    for tile in (13201, 16401, 11602, 11603, 23201, 26401, 21603, 31602, 31603):
         st = hassa_gpu(pdpkg, hassa_case, tile=tile)
         np.testing.assert_array_equal(st["u"], u)
         _, nonbc, elem = H.parity_norms(st["v"], v, bc_mask(hassa_case))
@@ -354,7 +353,7 @@ def test_hassa_matrix_core_contractions(pdpkg, hassa_case, hassa_oracle):
         assert nonbc < 1e-5
 
 
-@pytest.mark.parametrize("tile", [13201, 16401, 11602, 23201, 21603, 31602, 31603, 53201, 51602, 51603])
+@pytest.mark.parametrize("tile", [13201, 16401, 11602, 23201, 21603, 31602, 31603])
 def test_plate_matrix_core_breaking_matches_oracle(pdpkg, small_plate, small_plate_oracle, tile):
     say(f"plate 48x40, 300 steps, s0 = 0.005, matrix-core tile {tile}")
     pdh = pdpkg.pdhip
@@ -366,12 +365,10 @@ def test_plate_matrix_core_breaking_matches_oracle(pdpkg, small_plate, small_pla
     _, nonbc, _ = H.parity_norms(st["v"], v, mask)
     assert nonbc < 1e-10
     assert np.linalg.norm(st["u"] - u) < 1e-10 * np.linalg.norm(u)
-    if tile // 10000 == 2 or tile == 53201:
-        return  # matrix-core weights, and the rank form on 32-lane tiles, run in FP64 only
+    if tile // 10000 == 2:
+        return  # matrix-core weights run in FP64 only
     if tile // 10000 == 3:
         tile += 10000  # rank-form weights with matrix-core contractions in mixed precision
-    if tile // 10000 == 5:
-        tile -= 10000  # the same in mixed precision (FP32 matrix-core contractions)
     st, brg, _ = run_plate_gpu(pdh, small_plate, 0.005, 300, precision=1, tile=tile)
     np.testing.assert_array_equal(brg, br)
     _, nonbc, _ = H.parity_norms(st["v"], v, mask)
@@ -380,7 +377,7 @@ def test_plate_matrix_core_breaking_matches_oracle(pdpkg, small_plate, small_pla
 
 @pytest.mark.parametrize("precision,tile", [(0, 0), (1, 0), (0, 13201), (1, 13201), (0, 1603), (1, 11603),
                                             (0, 11603), (0, 21603), (0, 23201), (0, 31603), (1, 41603),
-                                            (1, 31602), (0, 51603), (0, 53201)])
+                                            (1, 31602)])
 def test_persistent_bond_kernel_bitwise(pdpkg, precision, tile):
     """The persistent bond kernel (each wave walks many family groups and
     prefetches the next one) gives the same bits as one wave per group."""
@@ -422,7 +419,7 @@ def test_delta_4015_family_of_48(pdpkg):
     assert stats.lanes_per_family * stats.bonds_per_lane >= 48
     _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
     assert nonbc < 1e-11
-    for tile in (16401, 1603, 11603, 21603, 26401, 31603, 51603, 56401):
+    for tile in (16401, 1603, 11603, 21603, 26401, 31603):
         st, _, _ = run_plate_gpu(pdpkg.pdhip, w, 0.0, 30, tile=tile)
         _, nonbc, _ = H.parity_norms(st["v"], v, w.code >= 0)
         assert nonbc < 1e-11
