@@ -98,9 +98,14 @@ __device__ __forceinline__ bool mazars_update(const BondArgs& a, double e11, dou
 }
 
 __device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
-// FP32: the median of (a, b, -inf) is min(a, b) in one v_med3_f32 (fminf
-// would first quiet each operand loaded from memory)
-__device__ __forceinline__ float rmin(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, -__builtin_inff()); }
+// FP32: one v_min_f32 (fminf would first quiet an operand loaded from LDS with
+// a v_max_f32; the operands here are finite)
+__device__ __forceinline__ float rmin(float a, float b)
+{
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 __device__ __forceinline__ double rmax(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }
 __device__ __forceinline__ double rpow(double a, double b) { return pow(a, b); }
