@@ -97,9 +97,14 @@ __device__ __forceinline__ bool mazars_update(const BondArgs& a, double e11, dou
     return true;
 }
 
-__device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
-// FP32: one v_min_f32 (fminf would first quiet an operand loaded from LDS with
-// a v_max_f32; the operands here are finite)
+// One v_min (fmin / fminf would first quiet an operand loaded from LDS with a
+// v_max of it with itself; the operands here are finite)
+__device__ __forceinline__ double rmin(double a, double b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 __device__ __forceinline__ float rmin(float a, float b)
 {
     float r;
@@ -195,6 +200,31 @@ template <int FPW>
 __device__ __forceinline__ void rec_put(double* rec, int k, int slot, int c, double v0, double v1)
 {
     *reinterpret_cast<double2*>(rec + rec_at<double, FPW>(k, slot, 2 * c)) = make_double2(v0, v1);
+}
+
+// Symmetric weight blocks (mixed-precision rank form, 16 lanes x 3 bonds):
+// lane t of a family holds w(16 S + t, 16 B + j), j = 0..15, of block (S, B),
+// S < B.  Since w(k, l) = w(l, k), block (B, S) is its transpose: the lanes
+// write their rows into the family's 16 x 17 LDS tile tt (rows of 17 floats,
+// so the column reads fall into distinct banks), read column t back as
+// w(16 B + t, 16 S + j) and add those weights times the records' values
+// rec[16 S + j][4 + lc] into slot B's matrix-core accumulator.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <int FPW, int S, int B, int FS>
+__device__ __forceinline__ void trx_block(const float (&w)[3][FS], const float* rec, float* tt, int t,
+                                          int slot, int lc, int mmax, v4f& acc)
+{
+    if (16 * B >= mmax) return;   // rows 16 B .. of every family in the wave are empty
+#pragma unroll
+    for (int j = 0; j < 16; ++j) tt[17 * t + j] = w[S][16 * B + j];
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const float om = tt[17 * j + t];
+        const float pa = rec[rec_at<float, FPW>(16 * S + j, slot, 4 + lc)];
+        acc = __builtin_amdgcn_mfma_f32_4x4x1f32(pa, om, acc, 0, 0, 0);
+    }
+    wave_sync();
 }
 
 // Values 4h .. 4h + 3 of record k.
@@ -1108,6 +1138,13 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     // two-bonds-per-lane tile, whose live set is its two weight rows)
     constexpr int GL = 4;
     R w[NC > 0 ? NC : 1][NC > 0 ? FS : 1];  // per-lane weight rows
+    // mixed-precision rank form with three bonds per lane: symmetric blocks
+    // (below); TRO is the transpose tile's offset in the wave's LDS, in floats,
+    // after the FS rows of records
+    constexpr bool TRX = MF32 && RF && BPL == 3 && NC == BPL;
+    constexpr int TRO = FS * (2 * FPW + 1) * 4;
+    static_assert(!TRX || TRO + FPW * 16 * 17 <= 2 * WaveLds<R, LPF, BPL, MF>::DOUBLES,
+                  "transpose tile within the wave's LDS");
     R D[BPL][4];
     v4f Dv[BPL];
 #pragma unroll
@@ -1168,6 +1205,71 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
             for (int c = 0; c < 4; ++c) D[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
+    } else if constexpr (TRX) {
+        // The weights are symmetric, w(k, l) = w(l, k): of a family's 48 x 48
+        // weight matrix, block (s, b) (rows 16 s .. 16 s + 15, columns 16 b ..)
+        // is evaluated for b >= s only and kept for contraction 2; block (b, s)
+        // is its transpose, formed through LDS in each contraction (trx_block):
+        // a third fewer weight evaluations and a third fewer kept weights.
+        // Columns of blocks 1 and 2 first, then the diagonal block of slot 0,
+        // then the transposed blocks.
+        float* tt = reinterpret_cast<float*>(rec) + TRO + F.slot * (16 * 17);
+#pragma unroll
+        for (int l0 = 16; l0 < FS; l0 += GL) {
+            if (l0 < mmax) {
+#pragma unroll
+                for (int l = l0; l < l0 + GL; ++l) {
+                    const R ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
+                    const R ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
+                    const R pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
+                    const R qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
+                    const R Pa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s) {
+                        if (s > (l >> 4)) continue;
+                        const R om = wt(s, l, ex, ey, pe, qe);
+                        w[s][l] = om;
+                        Dv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Pa, om, Dv[s], 0, 0, 0);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int l = l0; l < l0 + GL; ++l)
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s)
+                        if (s <= (l >> 4)) w[s][l] = R(0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        hook_a();
+#pragma unroll
+        for (int l0 = 0; l0 < 16; l0 += GL) {
+            if (l0 < mmax) {
+#pragma unroll
+                for (int l = l0; l < l0 + GL; ++l) {
+                    const R ex = rec[rec_at<R, FPW>(l, F.slot, 0)];
+                    const R ey = rec[rec_at<R, FPW>(l, F.slot, 1)];
+                    const R pe = rec[rec_at<R, FPW>(l, F.slot, 2)];
+                    const R qe = rec[rec_at<R, FPW>(l, F.slot, 3)];
+                    const R Pa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
+                    const R om = wt(0, l, ex, ey, pe, qe);
+                    w[0][l] = om;
+                    Dv[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(Pa, om, Dv[0], 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int l = l0; l < l0 + GL; ++l) w[0][l] = R(0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        trx_block<FPW, 0, 1>(w, rec, tt, F.t, F.slot, lc, mmax, Dv[1]);
+        trx_block<FPW, 0, 2>(w, rec, tt, F.t, F.slot, lc, mmax, Dv[2]);
+        trx_block<FPW, 1, 2>(w, rec, tt, F.t, F.slot, lc, mmax, Dv[2]);
+        hook_b();
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) D[s][c] = Dv[s][c];
     } else {
 #pragma unroll
     for (int l0 = 0; l0 < FS; l0 += GL) {
@@ -1352,6 +1454,32 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
             for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
+    } else if constexpr (TRX) {
+        // the kept blocks (s, b), b >= s, then the transposed ones
+        float* tt = reinterpret_cast<float*>(rec) + TRO + F.slot * (16 * 17);
+        v4f Sv[BPL];
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) Sv[s] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int l0 = 0; l0 < FS; l0 += GL) {
+            if (l0 < mmax) {
+#pragma unroll
+                for (int l = l0; l < l0 + GL; ++l) {
+                    const R Qa = rec[rec_at<R, FPW>(l, F.slot, 4 + lc)];
+#pragma unroll
+                    for (int s = 0; s < BPL; ++s)
+                        if (s <= (l >> 4)) Sv[s] = __builtin_amdgcn_mfma_f32_4x4x1f32(Qa, w[s][l], Sv[s], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        trx_block<FPW, 0, 1>(w, rec, tt, F.t, F.slot, lc, mmax, Sv[1]);
+        trx_block<FPW, 0, 2>(w, rec, tt, F.t, F.slot, lc, mmax, Sv[2]);
+        trx_block<FPW, 1, 2>(w, rec, tt, F.t, F.slot, lc, mmax, Sv[2]);
+#pragma unroll
+        for (int s = 0; s < BPL; ++s)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Sm[s][c] = Sv[s][c];
     } else {
     v4f Sv[BPL];
 #pragma unroll
