@@ -1221,8 +1221,10 @@ extern "C" int pd_autotune(pd_context* c, int steps)
         v.ms = ms / nsteps;
         return PD_OK;
     };
-    // pass 1: every variant for `steps` steps; pass 2: the four fastest again
-    // for 4x as many steps (short timings are noisy)
+    // pass 1: every variant for `steps` steps; pass 2: the four fastest again,
+    // in three interleaved rounds of 2x as many steps, each keeping its best
+    // round (short timings are noisy, and a power-held FP64 clock drifts over
+    // a tuning run, which would favour whichever variant is timed first)
     rc = PD_OK;
     for (Var& v : vars)
         if ((rc = time_var(v, steps))) break;
@@ -1230,11 +1232,17 @@ extern "C" int pd_autotune(pd_context* c, int steps)
     if (!rc && !vars.empty()) {
         std::sort(vars.begin(), vars.end(), [](const Var& a, const Var& b) { return a.ms < b.ms; });
         vars.resize(std::min<size_t>(4, vars.size()));
-        for (Var& v : vars)
-            if ((rc = time_var(v, 4 * steps))) break;
-        if (!rc)
+        std::vector<float> fastest(vars.size(), 0.f);
+        for (int round = 0; round < 3 && !rc; ++round)
+            for (size_t i = 0; i < vars.size(); ++i) {
+                if ((rc = time_var(vars[i], 2 * steps))) break;
+                fastest[i] = round == 0 ? vars[i].ms : std::min(fastest[i], vars[i].ms);
+            }
+        if (!rc) {
+            for (size_t i = 0; i < vars.size(); ++i) vars[i].ms = fastest[i];
             best = *std::min_element(vars.begin(), vars.end(),
                                      [](const Var& a, const Var& b) { return a.ms < b.ms; });
+        }
     }
     if (rc) save(true);
     use(rc ? keep : best);
