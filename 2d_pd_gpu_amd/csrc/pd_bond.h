@@ -1160,11 +1160,11 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     constexpr bool RW = NC < BPL;
     // neighbours per scheduling group of the contraction loops (each group's
     // record loads are issued together; 2 frees no registers in the FP64
-    // two-bonds-per-lane tile, whose live set is its two weight rows; the FP64
-    // rank form with three bonds per lane runs 8, measured -0.7 % against 4 on
-    // families of 48, where two bonds per lane lose 6 % with 8 on families of
-    // up to 28)
-    constexpr int GL = (RF && sizeof(R) == 8 && BPL == 3) ? 8 : 4;
+    // two-bonds-per-lane tile, whose live set is its two weight rows; the rank
+    // form with three bonds per lane runs 8, measured -0.7 % (FP64) and -1.4 %
+    // (mixed) against 4 on families of 48, where two bonds per lane lose 6 %
+    // with 8 on families of up to 28)
+    constexpr int GL = (RF && BPL == 3) ? 8 : 4;
     static_assert(FS % GL == 0 && (FS / 2 - GL) % GL == 0, "a load group ends halfway (hook_a)");
     R w[NC > 0 ? NC : 1][NC > 0 ? FS : 1];  // per-lane weight rows
     // mixed-precision rank form with three bonds per lane: symmetric blocks
