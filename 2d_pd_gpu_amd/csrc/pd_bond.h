@@ -1095,7 +1095,6 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     // run to the wave's largest family rounded up to 4 without guards
     const int mmax = wave_max_family<LPF>(F.M);
     wave_sync();
-    // the weight of slot s's bond against record l
     // The FP64 rank-form weights (n2 = 10) of slots s0 .. BPL - 1 against
     // record l, each step of the tenth power taken for every slot before the
     // next: the slots' chains run interleaved rather than one after the other
@@ -1118,6 +1117,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
 #pragma unroll
         for (int s = s0; s < BPL; ++s) om[s] = g4[s] * g2[s];
     };
+    // the weight of slot s's bond against record l
     auto wt = [&](int s, int l, R ex, R ey, R pe, R qe) -> R {
         if constexpr (RF) {
             const int bl = l >> 4;
