@@ -98,7 +98,9 @@ __device__ __forceinline__ bool mazars_update(const BondArgs& a, double e11, dou
 }
 
 // One v_min (fmin / fminf would first quiet an operand loaded from LDS with a
-// v_max of it with itself; the operands here are finite)
+// v_max of it with itself; the operands here are finite).  Inline asm gets no
+// hazard wait states from the compiler: only for operands computed by the
+// VALU or loaded from memory, never for matrix-core results.
 __device__ __forceinline__ double rmin(double a, double b)
 {
     double r;
@@ -333,16 +335,21 @@ __device__ __forceinline__ double rcp64(double x)
 __device__ __forceinline__ Geo<double> geometry64(const BondArgs& a, const BondSlot& S, double Ae)
 {
     Geo<double> g;
+    // branch-free: an inactive slot's math runs on a harmless value and is
+    // selected away, so a lane's slots schedule together
     const double l2 = S.x0 * S.x0 + S.x1 * S.x1;
-    const double il = S.act ? rsqrt64(l2) : 0.0;
+    const double ilr = rsqrt64(S.act ? l2 : 1.0);
+    const double il = S.act ? ilr : 0.0;
     const double len = l2 * il;
     g.len = len;
     g.ex = S.x0 * il;
     g.ey = S.x1 * il;
     g.hx = 0.5 * g.ex;
     g.hy = 0.5 * g.ey;
-    g.pe = S.act ? exp(Ae * len) : 0.0;
-    g.qe = S.act ? rcp64(g.pe) : 0.0;
+    const double pr = exp(Ae * (S.act ? len : 0.0));
+    g.pe = S.act ? pr : 0.0;
+    const double qr = rcp64(pr);
+    g.qe = S.act ? qr : 0.0;
     return g;
 }
 
@@ -782,8 +789,12 @@ __device__ __forceinline__ void mw_contract(R* lds, int lane, R n2, const HA& ho
                         g = mw_gram(U[e][s2], Lq[e][s1]);
                     } else {
                         const G4 gu = mw_gram(Lq[e][s2], U[e][s1]), gl = mw_gram(U[e][s2], Lq[e][s1]);
+                        // fmin, not the inline-asm rmin: its operands come
+                        // straight from the matrix cores, and only compiled
+                        // code gets the wait states an MFMA result needs
+                        // before a VALU reads it
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) g[r] = rmin(gu[r], gl[r]);
+                        for (int r = 0; r < 4; ++r) g[r] = fmin(R(gu[r]), R(gl[r]));
                     }
 #pragma unroll
                     for (int r = 0; r < 4; ++r) w[e][s1][r] = mw_pow<N2_IS_10>(R(g[r]), n2);
@@ -1072,10 +1083,16 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
     // order); their k-side factors U = p (1, e), L = q (1, e), r = q / p
     R Kk[RF ? BPL : 1][7];
     if constexpr (!MW) {
+    // rank-form records carry exp(+-A l / n2) and the P scaled by 2^-n2
+    const double Ag = RF ? (N2_IS_10 ? a.A * 0.1 : a.A / a.n2) : a.A;
+    if constexpr (sizeof(R) == 8) {
+        // every slot's FP64 geometry first: independent chains side by side
+#pragma unroll
+        for (int s = 0; s < BPL; ++s) g[s] = GeoOf<R>::get(a, S[s], Ag);
+    }
 #pragma unroll
     for (int s = 0; s < BPL; ++s) {
-        // rank-form records carry exp(+-A l / n2) and the P scaled by 2^-n2
-        g[s] = GeoOf<R>::get(a, S[s], RF ? (N2_IS_10 ? a.A * 0.1 : a.A / a.n2) : a.A);
+        if constexpr (sizeof(R) == 4) g[s] = GeoOf<R>::get(a, S[s], Ag);
         const int k = S[s].k;
         rec_put4<FPW>(rec, k, F.slot, 0, g[s].ex, g[s].ey, g[s].pe, g[s].qe);
         if constexpr (RF) rf_kside(g[s].ex, g[s].ey, g[s].pe, g[s].qe, Kk[s]);
