@@ -207,23 +207,29 @@ __device__ __forceinline__ void rec_put(double* rec, int k, int slot, int c, dou
 // Symmetric weight blocks (mixed-precision rank form, 16 lanes x 3 bonds):
 // lane t of a family holds w(16 S + t, 16 B + j), j = 0..15, of block (S, B),
 // S < B.  Since w(k, l) = w(l, k), block (B, S) is its transpose: the lanes
-// write their rows into the family's 16 x 17 LDS tile tt (rows of 17 floats,
-// so the column reads fall into distinct banks), read column t back as
+// write their rows into the family's 16 x 17 LDS tile at lds + tb (rows of
+// 17 floats, so the column reads fall into distinct banks), read column t back as
 // w(16 B + t, 16 S + j) and add those weights times the records' values
 // rec[16 S + j][4 + lc] into slot B's matrix-core accumulator.
 typedef float v4f __attribute__((ext_vector_type(4)));
 template <int FPW, int S, int B, int FS>
-__device__ __forceinline__ void trx_block(const float (&w)[3][FS], const float* rec, float* tt, int t,
-                                          int slot, int lc, int mmax, v4f& acc)
+__device__ __forceinline__ void trx_block(const float (&w)[3][FS], float* lds, int tb, int t, int slot, int lc,
+                                          int mmax, v4f& acc)
 {
     if (16 * B >= mmax) return;   // rows 16 B .. of every family in the wave are empty
+    // the lane's row and column starts in the tile, opaque to the compiler
+    // (an empty asm emits nothing) so that each access is one base register
+    // plus an immediate offset instead of an add per access
+    int rw = tb + 17 * t, cl = tb + t;
+    asm("" : "+v"(rw));
+    asm("" : "+v"(cl));
 #pragma unroll
-    for (int j = 0; j < 16; ++j) tt[17 * t + j] = w[S][16 * B + j];
+    for (int j = 0; j < 16; ++j) lds[rw + j] = w[S][16 * B + j];
     wave_sync();
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const float om = tt[17 * j + t];
-        const float pa = rec[rec_at<float, FPW>(16 * S + j, slot, 4 + lc)];
+        const float om = lds[cl + 17 * j];
+        const float pa = lds[rec_at<float, FPW>(16 * S + j, slot, 4 + lc)];
         acc = __builtin_amdgcn_mfma_f32_4x4x1f32(pa, om, acc, 0, 0, 0);
     }
     wave_sync();
@@ -1261,7 +1267,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
         // a third fewer weight evaluations and a third fewer kept weights.
         // Columns of blocks 1 and 2 first, then the diagonal block of slot 0,
         // then the transposed blocks.
-        float* tt = reinterpret_cast<float*>(rec) + TRO + F.slot * (16 * 17);
+        const int tb = TRO + F.slot * (16 * 17);   // the family's transpose tile
 #pragma unroll
         for (int l0 = 16; l0 < FS; l0 += GL) {
             if (l0 < mmax) {
@@ -1310,9 +1316,9 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-        trx_block<FPW, 0, 1>(w, rec, tt, F.t, F.slot, lc, mmax, Dv[1]);
-        trx_block<FPW, 0, 2>(w, rec, tt, F.t, F.slot, lc, mmax, Dv[2]);
-        trx_block<FPW, 1, 2>(w, rec, tt, F.t, F.slot, lc, mmax, Dv[2]);
+        trx_block<FPW, 0, 1>(w, rec, tb, F.t, F.slot, lc, mmax, Dv[1]);
+        trx_block<FPW, 0, 2>(w, rec, tb, F.t, F.slot, lc, mmax, Dv[2]);
+        trx_block<FPW, 1, 2>(w, rec, tb, F.t, F.slot, lc, mmax, Dv[2]);
         hook_b();
 #pragma unroll
         for (int s = 0; s < BPL; ++s)
@@ -1519,7 +1525,7 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             for (int c = 0; c < 4; ++c) Sm[s][c] = rec[rec_at<R, FPW>(S[s].k, F.slot, 4 + c)];
     } else if constexpr (TRX) {
         // the kept blocks (s, b), b >= s, then the transposed ones
-        float* tt = reinterpret_cast<float*>(rec) + TRO + F.slot * (16 * 17);
+        const int tb = TRO + F.slot * (16 * 17);   // the family's transpose tile
         v4f Sv[BPL];
 #pragma unroll
         for (int s = 0; s < BPL; ++s) Sv[s] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -1536,9 +1542,9 @@ __device__ __forceinline__ void bond_body(const BondArgs& a, FamLoad<R, BPL>& L,
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-        trx_block<FPW, 0, 1>(w, rec, tt, F.t, F.slot, lc, mmax, Sv[1]);
-        trx_block<FPW, 0, 2>(w, rec, tt, F.t, F.slot, lc, mmax, Sv[2]);
-        trx_block<FPW, 1, 2>(w, rec, tt, F.t, F.slot, lc, mmax, Sv[2]);
+        trx_block<FPW, 0, 1>(w, rec, tb, F.t, F.slot, lc, mmax, Sv[1]);
+        trx_block<FPW, 0, 2>(w, rec, tb, F.t, F.slot, lc, mmax, Sv[2]);
+        trx_block<FPW, 1, 2>(w, rec, tb, F.t, F.slot, lc, mmax, Sv[2]);
 #pragma unroll
         for (int s = 0; s < BPL; ++s)
 #pragma unroll
